@@ -1,0 +1,151 @@
+/* metapart_amd: MI355X (gfx950) implementation of MetaPrep's read-partitioning
+ * path behind a plain C ABI.
+ *
+ * The reference (vasupsu/MetaPart) has no library API: its boundary is three
+ * command-line tools, their files, and link-time "plugins" (getCanonKmers /
+ * init / getMmerPrefix).  Each entry point below names the reference code it
+ * replaces.  The host tools in metapart_amd/tools keep the reference's
+ * command lines and file layouts and call these functions.
+ *
+ * Conventions
+ *  - every call returns 0 on success and a negative MP_E* code on error;
+ *    nothing aborts.  mp_last_error() gives the message.
+ *  - host buffers are owned by the caller; device memory is owned by the ctx.
+ *  - a ctx is used by one host thread at a time; several ctxs may share a GPU.
+ *  - calls are stream-ordered on the ctx's stream and synchronous at return.
+ *  - read ids and labels are 32-bit: at most 4,294,967,294 reads per run.
+ */
+#ifndef METAPART_AMD_H
+#define METAPART_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_NBUCKETS 1048576u /* 4^10 m-mer buckets, m = 10 (genHistograms.c:105) */
+#define MP_NPART 16          /* output partitions (metapart.c:4016) */
+#define MP_PAIR_GATE 1000u   /* size gate of the pair step (metapart.c:4024) */
+
+enum {
+    MP_OK = 0,
+    MP_EINVAL = -1,   /* bad argument / malformed input */
+    MP_ENOMEM = -2,   /* host or device allocation failed, or HBM budget exceeded */
+    MP_EPLAN = -3,    /* degenerate range plan the reference cannot run either */
+    MP_ECOUNT = -4,   /* enumeration disagrees with the histogram sizes */
+    MP_EDEVICE = -5,  /* HIP runtime error */
+    MP_EIO = -6,      /* file error */
+    MP_ESTATE = -7    /* call out of order */
+};
+
+/* ---------------------------------------------------------------- host side */
+
+/* Chunk index: writes <files[0]>.idx with num_chunks+1 records of the
+ * reference's fileIndex layout (4,194,344 bytes each; unspecified bytes are 0).
+ * Replaces genFastqIdx main (genFastqIdx.c:142-376).  nPE counts file PAIRS. */
+int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const char *const *files, int nfiles);
+
+/* Range planning with one MPI rank: scan[S+1] and thread[S*T+1] bucket
+ * boundaries from the exclusive-prefix histogram hist[2^20+1].
+ * Replaces metapart.c:3720-3791.  Returns MP_EPLAN when a walk would write
+ * past its array or leaves a thread range empty or out of order. */
+int mp_plan_ranges(const uint64_t *hist, int S, int T, uint64_t *scan, uint64_t *thread);
+
+/* ---------------------------------------------------------------- device side */
+
+typedef struct mp_ctx mp_ctx;
+
+typedef struct {
+    int device;           /* HIP device ordinal */
+    int k;                /* k-mer length, 11..32 */
+    int min3;             /* 0: every k-mer; 1: three window minimizers per read */
+    uint64_t hbm_budget;  /* device bytes this ctx may hold; 0 = 90 % of free memory */
+    int sort_digit_bits;  /* radix digit width 8 or 11; 0 = autotuned default */
+    int sort_tile;        /* keys per sort tile (multiple of 1024); 0 = autotuned */
+} mp_cfg;
+
+typedef struct {
+    uint64_t tuples;          /* tuples enumerated in the scan */
+    uint64_t bad_labels;      /* labels marked BAD by the filter in this scan */
+    uint64_t zeroed_runs;     /* runs whose k-mer was zeroed (both sides branch) */
+    uint64_t evaluated_runs;  /* runs the filter evaluated */
+    int quirk_segments;       /* segments whose first run joined tuple 0's label */
+    int sort_passes;          /* radix passes executed */
+    float ms_enumerate, ms_sort, ms_filter, ms_union; /* device time */
+} mp_scan_stats;
+
+typedef struct {
+    uint64_t pair_unions;    /* unions made by the pair step */
+    uint64_t pair_candidates;/* pairs that needed the ordered pass */
+    uint64_t components;     /* roots after the pair step */
+    uint64_t largest;        /* size of the largest component */
+    float ms_device, ms_host;
+} mp_finish_stats;
+
+int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out);
+int mp_ctx_destroy(mp_ctx *ctx);
+const char *mp_last_error(const mp_ctx *ctx);
+/* Device bytes currently held by the ctx. */
+uint64_t mp_ctx_device_bytes(const mp_ctx *ctx);
+
+/* m-mer histograms of C chunks of FASTQ bytes.  Chunk c is
+ * fq[chunk_ofs[c] .. chunk_ofs[c+1]) and starts at a record.  counts receives
+ * C x (2^20+2) u32 (the kmerFreqCount of each chunk record, last two 0),
+ * reads_per_chunk C values; *total_kmers the sum.
+ * Replaces genHistograms.c:161-265 with the plugin getCanonKmers of
+ * countMmers.c:83-217 (min3 = 0) or countMmers_3minimizers.c:83-271. */
+int mp_count_chunks(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs, int C,
+                    uint32_t *counts, uint64_t *reads_per_chunk, uint64_t *total_kmers);
+
+/* Uploads the FASTQ bytes of a run (C chunks as above) and parses every
+ * record; chunk_first_id[c] is the read id of chunk c's first record
+ * (KmerIdx readId/readIdExtra), chunk_first_id[C] the read count.
+ * Reads stay resident across scans.
+ * Replaces loadChunk / getNextRead (metapart.c:296-500) and the scan-0
+ * parent initialisation (metapart.c:3933-3954). */
+int mp_load_reads(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs,
+                  const uint64_t *chunk_first_id, int C);
+
+/* Same, for FASTQ bytes already in device memory (not copied; must stay valid
+ * until the next mp_load_* or mp_ctx_destroy). */
+int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const uint64_t *chunk_ofs,
+                         const uint64_t *chunk_first_id, int C);
+
+/* One scan over buckets [lo, hi) split into T segments at seg_bounds[0..T]
+ * (thread ranges of the plan): enumerate, sort, filter, union, compress.
+ * Replaces generateKmersInScan + getCanonKmers of enumerateKmers.c:127-371 or
+ * enumerateKmers_3minimizers.c:129-474, sortKmerList, markMischiefKmers,
+ * unionFindPar2 / unionFind and printStats (metapart.c:1192-2800, 3227-3308).
+ * expected_tuples (may be 0 = unchecked) is the histogram's count for the
+ * scan; a mismatch returns MP_ECOUNT as the reference would overflow. */
+int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bounds, int T,
+            uint64_t expected_tuples, mp_scan_stats *stats);
+
+/* After the last scan: component sizes, the ordered pair step with the size
+ * gate, sizes again, and the 16-way partition plan W[0..16].
+ * Replaces metapart.c:4010-4056 and unionFindPeReads (2801-2867). */
+int mp_finish(mp_ctx *ctx, uint64_t *W, mp_finish_stats *stats);
+
+/* Final parent label of every read (read count entries). */
+int mp_get_parent(mp_ctx *ctx, uint64_t *parent);
+
+/* The last scan's sorted, filtered tuple array in the reference's 14-byte
+ * layout (enumerateKmers.c:352-365: k-mer u64, adj u8, id/UINT32_MAX u8,
+ * id%UINT32_MAX u32).  Debug and parity only. */
+int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n);
+
+/* Partitioned FASTQ writer.  Builds the 16*T output streams on the device
+ * (record bytes copied verbatim, pairs duplicated when their roots differ)
+ * and writes OUT/Rank0/FromTid<t>_ToRank<p>.paired.fastq.
+ * il_start[c] < 0: chunk c has no interleaved part; otherwise the byte offset
+ * (within the chunk) from which records pair up, as loadChunk leaves it.
+ * out_prefix NULL: build the streams on the device only (timing).
+ * Replaces writeOutput3 (metapart.c:956-1066). */
+int mp_write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
+                        uint64_t *bytes_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,239 @@
+// C-ABI entry points of libmetapart_amd (declared in include/metapart_amd.h).
+#include <cstring>
+#include <vector>
+
+#include "mp_internal.h"
+
+using namespace mp;
+
+extern "C" {
+
+int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
+{
+    if (!cfg || !out) return MP_EINVAL;
+    *out = nullptr;
+    if (cfg->k <= 10 || cfg->k > 32) return MP_EINVAL;
+    mp_ctx *ctx = new (std::nothrow) mp_ctx();
+    if (!ctx) return MP_ENOMEM;
+    ctx->cfg = *cfg;
+    ctx->device = cfg->device;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+        (void)hipGetLastError();
+        delete ctx;
+        return MP_EDEVICE;
+    }
+    if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MP_EDEVICE;
+    }
+    for (auto &e : ctx->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return MP_EDEVICE;
+        }
+    size_t fre = 0, tot = 0;
+    (void)hipMemGetInfo(&fre, &tot);
+    ctx->budget = cfg->hbm_budget ? cfg->hbm_budget : (uint64_t)(fre * 0.9);
+    *out = ctx;
+    return 0;
+}
+
+int mp_ctx_destroy(mp_ctx *ctx)
+{
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
+                        &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
+                        &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
+                        &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
+                        &ctx->iaux[0], &ctx->iaux[1]};
+    for (DevArray *a : arrs) release(ctx, *a);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+const char *mp_last_error(const mp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+uint64_t mp_ctx_device_bytes(const mp_ctx *ctx) { return ctx ? ctx->held : 0; }
+
+static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_first_id, int C)
+{
+    if (C <= 0 || !chunk_ofs) return ctx->fail(MP_EINVAL, "need at least one chunk");
+    if (chunk_ofs[0] != 0 || chunk_ofs[C] != ctx->fq_bytes)
+        return ctx->fail(MP_EINVAL, "chunks must cover the FASTQ bytes exactly");
+    for (int c = 0; c < C; c++)
+        if (chunk_ofs[c + 1] < chunk_ofs[c]) return ctx->fail(MP_EINVAL, "chunk offsets must not decrease");
+    ctx->have_scan = false;
+    ctx->finished = false;
+    ctx->scans_done = 0;
+    ctx->last_n = 0;
+    MP_TRY(parse_reads(ctx, chunk_ofs, chunk_first_id, C));
+    ctx->chunk_ofs.assign(chunk_ofs, chunk_ofs + C + 1);
+    const uint64_t N = ctx->N;
+    MP_TRY(ensure(ctx, ctx->parent, N * sizeof(uint32_t)));
+    MP_TRY(ensure(ctx, ctx->bad, N + 16));
+    MP_TRY(ensure(ctx, ctx->filtered, N + 16));
+    MP_TRY(ensure(ctx, ctx->sizes, N * sizeof(uint32_t)));
+    return 0;
+}
+
+static int upload_fq(mp_ctx *ctx, const char *fq, uint64_t nbytes)
+{
+    MP_TRY(ensure(ctx, ctx->fq, nbytes + 64));
+    MP_HIP(hipMemcpyAsync(ctx->fq.p, fq, nbytes, hipMemcpyHostToDevice, ctx->stream));
+    MP_TRY(fill_u32(ctx, (uint8_t *)ctx->fq.p + ((nbytes + 3) & ~3ull), 0, 8));
+    ctx->d_fq = (const uint8_t *)ctx->fq.p;
+    ctx->fq_owned = true;
+    ctx->fq_bytes = nbytes;
+    return 0;
+}
+
+int mp_count_chunks(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs, int C, uint32_t *counts,
+                    uint64_t *reads_per_chunk, uint64_t *total_kmers)
+{
+    if (!ctx || !fq || !counts) return MP_EINVAL;
+    MP_HIP(hipSetDevice(ctx->device));
+    MP_TRY(upload_fq(ctx, fq, nbytes));
+    MP_TRY(load_common(ctx, chunk_ofs, nullptr, C));
+    const uint64_t words = (uint64_t)C * (MP_NBUCKETS + 2);
+    MP_TRY(ensure(ctx, ctx->scratch, words * sizeof(uint32_t) + 4096));
+    uint32_t *d_counts = (uint32_t *)ctx->scratch.p;
+    MP_TRY(count_chunks_device(ctx, ctx->chunk_ids.data(), C, d_counts));
+    MP_HIP(hipMemcpyAsync(counts, d_counts, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t tot = 0;
+    for (uint64_t i = 0; i < words; i++) tot += counts[i];
+    if (total_kmers) *total_kmers = tot;
+    if (reads_per_chunk)
+        for (int c = 0; c < C; c++) reads_per_chunk[c] = ctx->chunk_ids[c + 1] - ctx->chunk_ids[c];
+    return 0;
+}
+
+int mp_load_reads(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs,
+                  const uint64_t *chunk_first_id, int C)
+{
+    if (!ctx || !fq) return MP_EINVAL;
+    MP_HIP(hipSetDevice(ctx->device));
+    MP_TRY(upload_fq(ctx, fq, nbytes));
+    return load_common(ctx, chunk_ofs, chunk_first_id, C);
+}
+
+int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const uint64_t *chunk_ofs,
+                         const uint64_t *chunk_first_id, int C)
+{
+    if (!ctx || !d_fq) return MP_EINVAL;
+    MP_HIP(hipSetDevice(ctx->device));
+    ctx->d_fq = (const uint8_t *)d_fq;
+    ctx->fq_owned = false;
+    ctx->fq_bytes = nbytes;
+    return load_common(ctx, chunk_ofs, chunk_first_id, C);
+}
+
+int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bounds, int T, uint64_t expected_tuples,
+            mp_scan_stats *stats)
+{
+    if (!ctx || !seg_bounds || T <= 0) return MP_EINVAL;
+    if (ctx->N == 0) return ctx->fail(MP_ESTATE, "mp_scan before mp_load_reads");
+    if (s != ctx->scans_done) return ctx->fail(MP_ESTATE, "scan %d out of order (next is %d)", s, ctx->scans_done);
+    if (seg_bounds[0] != lo || seg_bounds[T] != hi) return ctx->fail(MP_EINVAL, "segments must span [lo, hi)");
+    for (int t = 0; t < T; t++)
+        if (!(seg_bounds[t] < seg_bounds[t + 1])) return ctx->fail(MP_EPLAN, "empty or unordered thread range %d", t);
+    MP_HIP(hipSetDevice(ctx->device));
+    mp_scan_stats st;
+    memset(&st, 0, sizeof(st));
+    const uint64_t N = ctx->N;
+    if (s == 0) {  // metapart.c:3933-3954
+        MP_TRY(iota_u32(ctx, (uint32_t *)ctx->parent.p, N));
+        MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
+        MP_TRY(fill_u32(ctx, ctx->filtered.p, 0, (N + 3) / 4));
+    }
+    hipEventRecord(ctx->ev[0], ctx->stream);
+    uint64_t n = 0;
+    MP_TRY(enumerate_scan(ctx, s, lo, hi, &n));
+    hipEventRecord(ctx->ev[1], ctx->stream);
+    if (expected_tuples && n != expected_tuples)
+        return ctx->fail(MP_ECOUNT, "scan %d enumerated %llu tuples, the histogram sized %llu", s,
+                         (unsigned long long)n, (unsigned long long)expected_tuples);
+    int passes = 0;
+    MP_TRY(radix_sort_tuples(ctx, n, &passes));
+    hipEventRecord(ctx->ev[7], ctx->stream);
+    MP_TRY(filter_union_scan(ctx, n, seg_bounds, T, &st));
+    st.tuples = n;
+    st.sort_passes = passes;
+    hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
+    hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
+    if (stats) *stats = st;
+    ctx->last_n = n;
+    ctx->have_scan = true;
+    ctx->scans_done++;
+    return 0;
+}
+
+int mp_finish(mp_ctx *ctx, uint64_t *W, mp_finish_stats *stats)
+{
+    if (!ctx || !W) return MP_EINVAL;
+    if (ctx->scans_done == 0) return ctx->fail(MP_ESTATE, "mp_finish before any scan");
+    MP_HIP(hipSetDevice(ctx->device));
+    mp_finish_stats st;
+    memset(&st, 0, sizeof(st));
+    MP_TRY(finish_components(ctx, W, &st));
+    if (stats) *stats = st;
+    ctx->finished = true;
+    return 0;
+}
+
+int mp_get_parent(mp_ctx *ctx, uint64_t *parent)
+{
+    if (!ctx || !parent) return MP_EINVAL;
+    if (ctx->N == 0) return ctx->fail(MP_ESTATE, "no reads loaded");
+    MP_HIP(hipSetDevice(ctx->device));
+    std::vector<uint32_t> p(ctx->N);
+    MP_HIP(hipMemcpyAsync(p.data(), ctx->parent.p, ctx->N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t i = 0; i < ctx->N; i++) parent[i] = p[i];
+    return 0;
+}
+
+int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n)
+{
+    if (!ctx || !n) return MP_EINVAL;
+    if (!ctx->have_scan) return ctx->fail(MP_ESTATE, "no scan to dump");
+    *n = ctx->last_n;
+    if (!dst) return 0;
+    if (cap < ctx->last_n) return ctx->fail(MP_EINVAL, "dump buffer holds %llu tuples, need %llu",
+                                            (unsigned long long)cap, (unsigned long long)ctx->last_n);
+    MP_HIP(hipSetDevice(ctx->device));
+    const uint64_t m = ctx->last_n;
+    std::vector<uint64_t> k(m);
+    std::vector<uint32_t> l(m);
+    std::vector<uint8_t> a(m);
+    MP_HIP(hipMemcpyAsync(k.data(), ctx->keys[ctx->cur].p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(l.data(), ctx->lab[ctx->cur].p, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(a.data(), ctx->adj[ctx->cur].p, m, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t i = 0; i < m; i++) {  // enumerateKmers.c:352-365 layout
+        uint8_t *t = dst + 14 * i;
+        memcpy(t, &k[i], 8);
+        t[8] = a[i];
+        t[9] = 0;  // id / UINT32_MAX: ids are 32-bit here
+        memcpy(t + 10, &l[i], 4);
+    }
+    return 0;
+}
+
+int mp_write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
+                        uint64_t *bytes_out)
+{
+    if (!ctx || !W || T <= 0) return MP_EINVAL;
+    if (!ctx->finished) return ctx->fail(MP_ESTATE, "mp_write_partitions before mp_finish");
+    if (ctx->C % T) return ctx->fail(MP_EINVAL, "T must divide the chunk count");
+    MP_HIP(hipSetDevice(ctx->device));
+    return write_partitions(ctx, out_prefix, T, W, il_start, bytes_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,456 @@
+// Frequency filter, union-find and the post-scan partition plan.
+//
+// State per read (32-bit ids): parent[i], bad[i] (the reference's BAD
+// sentinel 0xBADBADBAD in parent), filtered[i] (filteredReads).  A BAD read
+// keeps parent[i] == i: in the reference a label is only ever marked BAD while
+// it is a root, and findRoot stops at a node whose parent is BAD, which is the
+// same as stopping at a self-parent (unionfind.h:19-55).
+//
+// Filter (markMischiefKmers, metapart.c:2270-2519): one lane per tuple; the
+// lane holding a run's first tuple evaluates the run unless it is the last run
+// of its segment.  Zeroing is recorded in a bitmap and applied afterwards, so
+// no lane sees a zeroed key while another lane is still finding run starts.
+//
+// Union (unionFindPar2 + unionFind, metapart.c:2520-2800): the reference
+// star-unions the non-BAD labels of each run and always hooks the smaller root
+// under the larger one, so every component's root is its largest id.  Here
+// each run start does the same with a lock-free union-find: find with path
+// halving, hook by compare-and-swap of the smaller root's parent.  Since
+// parent[x] >= x always holds, the final roots are the component maxima
+// whatever order the lanes run in.
+#include "mp_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace mp {
+
+struct GParams {
+    const uint64_t *keys;
+    const uint8_t *adj;
+    const uint32_t *lab;
+    uint64_t n;
+    const uint64_t *seg;  // T+1 tuple positions
+    int T;
+    uint8_t *bad;
+    uint8_t *filtered;
+    uint32_t *zbits;
+    uint32_t *parent;
+    unsigned long long *stats;  // [0] evaluated runs, [1] zeroed runs
+};
+
+__device__ __forceinline__ void seg_of(const GParams &G, uint64_t i, uint64_t &s0, uint64_t &s1)
+{
+    int lo = 0, hi = G.T - 1;  // last t with seg[t] <= i
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (G.seg[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    s0 = G.seg[lo];
+    s1 = G.seg[lo + 1];
+}
+
+__global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, int bshift,
+                               const uint64_t *__restrict__ bounds, int T, uint64_t *__restrict__ pos)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > T) return;
+    const uint64_t target = bounds[t];
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((keys[mid] >> bshift) < target) lo = mid + 1; else hi = mid;
+    }
+    pos[t] = lo;
+}
+
+__global__ void filter_kernel(GParams G)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    uint64_t s0, s1;
+    seg_of(G, i, s0, s1);
+    const uint64_t k = G.keys[i];
+    if (i != s0 && G.keys[i - 1] == k) return;  // not a run start
+    uint64_t j = i + 1;
+    while (j < s1 && G.keys[j] == k) j++;
+    if (j == s1) return;  // the last run of a segment is never evaluated
+    atomicAdd(&G.stats[0], 1ull);
+    uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t t = i; t < j; t++) {
+        const uint8_t a = G.adj[t];
+        in[a & 7]++;
+        out[(a >> 3) & 7]++;
+    }
+    int nIn = 0, nOut = 0, maxIn = 0, maxOut = 0;
+    uint32_t mIn = 0, mOut = 0, sIn = 0, sOut = 0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {  // metapart.c:2300-2333, ties keep the first maximum
+        if (in[c]) {
+            if (in[c] > mIn) { sIn = mIn; mIn = in[c]; maxIn = c; }
+            else if (in[c] > sIn) sIn = in[c];
+            nIn++;
+        }
+        if (out[c]) {
+            if (out[c] > mOut) { sOut = mOut; mOut = out[c]; maxOut = c; }
+            else if (out[c] > sOut) sOut = out[c];
+            nOut++;
+        }
+    }
+    if (!(nIn > 1 || nOut > 1)) return;
+    if (nIn > 1 && nOut > 1) {  // both sides branch: the k-mer is zeroed
+        atomicAdd(&G.stats[1], 1ull);
+        for (uint64_t t = i; t < j;) {
+            const uint64_t w = t >> 5;
+            const uint64_t e = ((w + 1) << 5) < j ? ((w + 1) << 5) : j;
+            const uint32_t nb = (uint32_t)(e - t);
+            const uint32_t m = (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1)) << (t & 31);
+            atomicOr(&G.zbits[w], m);
+            t = e;
+        }
+        return;
+    }
+    // (float)smax / degree compared with the double 0.2 (metapart.c:2356-2359, 2419)
+    const float deg = (float)(int)(j - i);
+    const bool ratio = (double)__fdiv_rn((float)sIn, deg) > 0.2 || (double)__fdiv_rn((float)sOut, deg) > 0.2;
+    for (uint64_t t = i; t < j; t++) {
+        const uint8_t a = G.adj[t];
+        const int bi = a & 7, ai = (a >> 3) & 7;
+        if ((nIn > 1 && bi != maxIn) || (nOut > 1 && ai != maxOut) || ratio) {
+            const uint32_t L = G.lab[t];
+            G.bad[L] = 1;
+            G.filtered[L] = 1;
+        }
+    }
+}
+
+__global__ void apply_zero_kernel(uint64_t *__restrict__ keys, const uint32_t *__restrict__ zbits, uint64_t nwords)
+{
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint32_t m = zbits[w];
+    while (m) {
+        const int b = __ffs(m) - 1;
+        keys[(w << 5) + b] = 0;
+        m &= m - 1;
+    }
+}
+
+// metapart.c:2500-2517: a read filtered in any scan is BAD again after the filter
+__global__ void refilter_kernel(uint8_t *__restrict__ bad, const uint8_t *__restrict__ filtered,
+                                uint32_t *__restrict__ parent, uint64_t N, unsigned long long *__restrict__ nbad)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool b = false;
+    if (i < N) {
+        if (filtered[i] && !bad[i]) {
+            bad[i] = 1;
+            parent[i] = (uint32_t)i;
+        }
+        b = bad[i] != 0;
+    }
+    const uint64_t m = __ballot(b);
+    if (__lane_id() == 0 && m) atomicAdd(nbad, (unsigned long long)__popcll(m));
+}
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t *parent, uint32_t x)
+{
+    for (;;) {
+        const uint32_t p = ld_agent(&parent[x]);
+        if (p == x) return x;
+        const uint32_t gp = ld_agent(&parent[p]);
+        if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void uf_union(uint32_t *parent, uint32_t a, uint32_t b)
+{
+    for (;;) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a < b) { const uint32_t t = a; a = b; b = t; }
+        // hook the smaller root under the larger (unionfind.h:58-69 with u > v)
+        uint32_t expect = b;
+        if (__hip_atomic_compare_exchange_strong(&parent[b], &expect, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+__global__ void union_kernel(GParams G)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    uint64_t s0, s1;
+    seg_of(G, i, s0, s1);
+    const uint64_t k = G.keys[i];
+    if (i != s0 && G.keys[i - 1] == k) return;
+    if (k == 0) return;  // zeroed run or the all-A k-mer (metapart.c:2574)
+    uint64_t j = i + 1;
+    while (j < s1 && G.keys[j] == k) j++;
+    uint64_t c = i;
+    while (c < j && G.bad[G.lab[c]]) c++;
+    if (c >= j) return;
+    const uint32_t a = G.lab[c];
+    for (uint64_t t = c + 1; t < j; t++) {
+        const uint32_t b = G.lab[t];
+        if (b != a && !G.bad[b]) uf_union(G.parent, a, b);
+    }
+}
+
+// First surviving tuple of each segment (non-BAD label, non-zero key).
+__global__ void first_alive_kernel(GParams G, uint64_t *__restrict__ first)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G.T) return;
+    uint64_t i = G.seg[t];
+    const uint64_t e = G.seg[t + 1];
+    while (i < e && (G.bad[G.lab[i]] || G.keys[i] == 0)) i++;
+    first[t] = i < e ? i : ~0ull;
+}
+
+__global__ void quirk_union_kernel(uint32_t *parent, uint32_t a, uint32_t b) { uf_union(parent, a, b); }
+__global__ void quirk_fix_kernel(uint32_t *parent, uint8_t *bad, uint32_t L0)
+{
+    if (bad[L0] && parent[L0] != L0) bad[L0] = 0;
+}
+
+__global__ void compress_kernel(uint32_t *parent, const uint8_t *__restrict__ bad, uint64_t N)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || bad[i]) return;
+    uint32_t x = (uint32_t)i;
+    for (;;) {
+        const uint32_t p = parent[x];
+        if (p == x) break;
+        x = p;
+    }
+    parent[i] = x;
+}
+
+__global__ void sizes_kernel(const uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes, uint64_t N)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) atomicAdd(&sizes[parent[i]], 1u);
+}
+
+__global__ void root_weight_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
+                                   uint32_t *__restrict__ w, uint64_t N, unsigned long long *__restrict__ st)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool root = false;
+    uint32_t s = 0;
+    if (i < N) {
+        root = parent[i] == (uint32_t)i;
+        s = root ? sizes[i] : 0;
+        w[i] = s;
+    }
+    const uint64_t m = __ballot(root);
+    if (__lane_id() == 0 && m) atomicAdd(&st[0], (unsigned long long)__popcll(m));
+    if (root) atomicMax(&st[1], (unsigned long long)s);
+}
+
+__global__ void plan_kernel(const uint64_t *__restrict__ excl, uint64_t N, uint64_t *__restrict__ W)
+{
+    const int p = threadIdx.x;
+    if (p > MP_NPART) return;
+    if (p == 0) { W[0] = 0; return; }
+    if (p == MP_NPART) { W[p] = N; return; }
+    const uint64_t target = (uint64_t)p * (N / MP_NPART);  // metapart.c:4036-4053
+    uint64_t lo = 0, hi = N;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (excl[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    W[p] = lo;
+}
+
+static inline unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st)
+{
+    const uint64_t N = ctx->N;
+    const int bshift = 2 * (ctx->cfg.k - 10);
+    uint64_t *keys = (uint64_t *)ctx->keys[ctx->cur].p;
+    const uint32_t *lab = (const uint32_t *)ctx->lab[ctx->cur].p;
+    const uint8_t *adj = (const uint8_t *)ctx->adj[ctx->cur].p;
+    const uint64_t nwords = (n + 31) / 32 + 1;
+    MP_TRY(ensure(ctx, ctx->zbits, nwords * sizeof(uint32_t)));
+    MP_TRY(fill_u32(ctx, ctx->zbits.p, 0, nwords));
+    // small device scratch: bounds, positions, stats, first-alive
+    uint64_t *d_small = nullptr;
+    const size_t small_words = 3 * (size_t)(T + 1) + 16;
+    MP_HIP(hipMallocAsync((void **)&d_small, small_words * sizeof(uint64_t), ctx->stream));
+    uint64_t *d_bounds = d_small, *d_seg = d_small + (T + 1), *d_first = d_small + 2 * (T + 1);
+    unsigned long long *d_stats = (unsigned long long *)(d_small + 3 * (T + 1));
+    MP_TRY(fill_u32(ctx, d_stats, 0, 32));
+    MP_HIP(hipMemcpyAsync(d_bounds, seg_bounds, sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(seg_pos_kernel, dim3(nblk(T + 1)), dim3(256), 0, ctx->stream, (const uint64_t *)keys, n, bshift,
+                       (const uint64_t *)d_bounds, T, d_seg);
+    MP_HIP(hipGetLastError());
+    std::vector<uint64_t> segpos(T + 1);
+    MP_HIP(hipMemcpyAsync(segpos.data(), d_seg, sizeof(uint64_t) * (T + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (segpos[0] != 0 || segpos[T] != n) {
+        MP_HIP(hipFreeAsync(d_small, ctx->stream));
+        return ctx->fail(MP_EPLAN, "tuples outside the scan's thread ranges");
+    }
+    GParams G;
+    G.keys = keys;
+    G.adj = adj;
+    G.lab = lab;
+    G.n = n;
+    G.seg = d_seg;
+    G.T = T;
+    G.bad = (uint8_t *)ctx->bad.p;
+    G.filtered = (uint8_t *)ctx->filtered.p;
+    G.zbits = (uint32_t *)ctx->zbits.p;
+    G.parent = (uint32_t *)ctx->parent.p;
+    G.stats = d_stats;
+    hipEventRecord(ctx->ev[2], ctx->stream);
+    if (n) {
+        hipLaunchKernelGGL(filter_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, G);
+        MP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(apply_zero_kernel, dim3(nblk(nwords)), dim3(256), 0, ctx->stream, keys,
+                           (const uint32_t *)ctx->zbits.p, nwords);
+        MP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(refilter_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.bad,
+                       (const uint8_t *)G.filtered, G.parent, N, d_stats + 2);
+    MP_HIP(hipGetLastError());
+    hipEventRecord(ctx->ev[3], ctx->stream);
+    // first-run quirk (metapart.c:2539-2616): each segment starts with the
+    // global last k-mer as the "current" k-mer and tuple 0's label as centre
+    int quirks = 0;
+    if (n) {
+        hipLaunchKernelGGL(first_alive_kernel, dim3(nblk(T)), dim3(256), 0, ctx->stream, G, d_first);
+        MP_HIP(hipGetLastError());
+        std::vector<uint64_t> first(T);
+        uint64_t lastK = 0;
+        uint32_t L0 = 0;
+        MP_HIP(hipMemcpyAsync(first.data(), d_first, sizeof(uint64_t) * T, hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipMemcpyAsync(&lastK, keys + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipMemcpyAsync(&L0, lab, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        uint8_t L0bad = 0;
+        for (int t = 0; t < T; t++) {
+            if (first[t] == ~0ull) continue;
+            uint64_t kf = 0;
+            uint32_t lf = 0;
+            MP_HIP(hipMemcpyAsync(&kf, keys + first[t], sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(&lf, lab + first[t], sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(&L0bad, G.bad + L0, 1, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipStreamSynchronize(ctx->stream));
+            if (kf != lastK) continue;
+            if (L0bad) {
+                MP_HIP(hipFreeAsync(d_small, ctx->stream));
+                return ctx->fail(MP_EPLAN, "first-run union quirk with a filtered tuple-0 label (segment %d)", t);
+            }
+            quirks++;
+            if (lf != L0) {
+                hipLaunchKernelGGL(quirk_union_kernel, dim3(1), dim3(1), 0, ctx->stream, G.parent, L0, lf);
+                MP_HIP(hipGetLastError());
+            }
+        }
+        hipLaunchKernelGGL(union_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, G);
+        MP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.parent, (const uint8_t *)G.bad, N);
+    MP_HIP(hipGetLastError());
+    hipEventRecord(ctx->ev[4], ctx->stream);
+    unsigned long long hs[4] = {0, 0, 0, 0};
+    MP_HIP(hipMemcpyAsync(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipFreeAsync(d_small, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (st) {
+        st->evaluated_runs = hs[0];
+        st->zeroed_runs = hs[1];
+        st->bad_labels = hs[2];
+        st->quirk_segments = quirks;
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, ctx->ev[2], ctx->ev[3]);
+        hipEventElapsedTime(&b, ctx->ev[3], ctx->ev[4]);
+        st->ms_filter = a;
+        st->ms_union = b;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- finish
+// Sequential pair step (unionFindPeReads, metapart.c:2801-2867) on the host.
+static uint32_t hfind(uint32_t *parent, uint32_t x)
+{
+    while (parent[x] != x) {
+        const uint32_t p = parent[x], gp = parent[p];
+        parent[x] = gp;
+        x = gp;
+    }
+    return x;
+}
+
+int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
+{
+    const uint64_t N = ctx->N;
+    uint32_t *parent = (uint32_t *)ctx->parent.p;
+    uint32_t *sizes = (uint32_t *)ctx->sizes.p;
+    hipEventRecord(ctx->ev[5], ctx->stream);
+    // sizes after the last scan (metapart.c:4010-4021)
+    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, (const uint8_t *)ctx->bad.p, N);
+    MP_TRY(fill_u32(ctx, sizes, 0, N));
+    hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent, sizes, N);
+    MP_HIP(hipGetLastError());
+    std::vector<uint32_t> hp(N), hs(N);
+    MP_HIP(hipMemcpyAsync(hp.data(), parent, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(hs.data(), sizes, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t unions = 0, cand = 0;
+    for (uint64_t i = 0; i + 1 < N; i += 2) {
+        const uint32_t a = hfind(hp.data(), (uint32_t)i), b = hfind(hp.data(), (uint32_t)(i + 1));
+        if (a == b) continue;
+        if (hs[a] <= MP_PAIR_GATE || hs[b] <= MP_PAIR_GATE) {
+            cand++;
+            unions++;
+            if (a > b) { hp[b] = a; hs[a] += hs[b]; hs[b] = 0; }
+            else { hp[a] = b; hs[b] += hs[a]; hs[a] = 0; }
+        }
+    }
+    MP_HIP(hipMemcpyAsync(parent, hp.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+    // every BAD read was reset to itself by the pair step
+    MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
+    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, (const uint8_t *)ctx->bad.p, N);
+    MP_TRY(fill_u32(ctx, sizes, 0, N));
+    hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent, sizes, N);
+    MP_HIP(hipGetLastError());
+    // partition plan over root ids in ascending order
+    MP_TRY(ensure(ctx, ctx->wg_off, (N + 1) * sizeof(uint64_t)));
+    MP_TRY(ensure(ctx, ctx->wg_cnt, N * sizeof(uint32_t)));
+    uint64_t *d_small = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_small, 64 * sizeof(uint64_t), ctx->stream));
+    MP_TRY(fill_u32(ctx, d_small, 0, 128));
+    hipLaunchKernelGGL(root_weight_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+                       (const uint32_t *)sizes, (uint32_t *)ctx->wg_cnt.p, N, (unsigned long long *)(d_small + 32));
+    MP_HIP(hipGetLastError());
+    MP_TRY(scan_u32(ctx, (const uint32_t *)ctx->wg_cnt.p, (uint64_t *)ctx->wg_off.p, N));
+    hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(64), 0, ctx->stream, (const uint64_t *)ctx->wg_off.p, N, d_small);
+    MP_HIP(hipGetLastError());
+    uint64_t res[64];
+    MP_HIP(hipMemcpyAsync(res, d_small, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipFreeAsync(d_small, ctx->stream));
+    hipEventRecord(ctx->ev[6], ctx->stream);
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p <= MP_NPART; p++) W[p] = res[p];
+    if (st) {
+        st->pair_unions = unions;
+        st->pair_candidates = cand;
+        st->components = res[32];
+        st->largest = res[33];
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]);
+        st->ms_device = ms;
+    }
+    return 0;
+}
+
+}  // namespace mp

@@ -1,0 +1,139 @@
+// Internal declarations of the metapart_amd HIP library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/metapart_amd.h"
+
+namespace mp {
+
+constexpr int kWave = 64;
+constexpr uint32_t kBad = 1;  // bit of bad[]: label marked by the frequency filter
+
+// ---------------------------------------------------------------- device helpers
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    const unsigned lane = __lane_id();
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// Reverse-complement of a 2-bit packed k-mer (first base most significant,
+// A0 C1 G2 T3, complement = 3 - code): reverse the 2-bit groups of ~x.
+__device__ __forceinline__ uint64_t revcomp(uint64_t x, int K)
+{
+    x = ~x;
+    x = __builtin_bitreverse64(x);
+    x = ((x >> 1) & 0x5555555555555555ull) | ((x & 0x5555555555555555ull) << 1);
+    return x >> (64 - 2 * K);
+}
+
+// ---------------------------------------------------------------- context
+struct DevArray {
+    void *p = nullptr;
+    uint64_t bytes = 0;
+};
+
+}  // namespace mp
+
+struct mp_ctx {
+    mp_cfg cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8]{};
+    std::string err;
+    uint64_t budget = 0, held = 0;
+    uint16_t epoch = 0;  // look-back status epoch of the radix sort
+
+    // reads
+    const uint8_t *d_fq = nullptr;  // FASTQ bytes (owned if fq_owned)
+    bool fq_owned = false;
+    uint64_t fq_bytes = 0;
+    uint64_t N = 0;  // reads
+    int C = 0;       // chunks
+    uint64_t total_bases = 0;
+
+    // per-read arrays
+    mp::DevArray fq, rec_off, seq_off, seq_len, base_off, pos_off, packed, nmask;
+    // graph state
+    mp::DevArray parent, bad, filtered, sizes;
+    // scan buffers (ping-pong)
+    mp::DevArray keys[2], lab[2], adj[2], zbits;
+    mp::DevArray wg_cnt, wg_off, status, scratch, hist;
+    // writer
+    mp::DevArray out, items[2], ilens[2], iaux[2];
+    std::vector<uint64_t> chunk_ofs;  // byte offset of each chunk (C+1)
+    int cur = 0;               // buffer holding the last scan's sorted tuples
+    uint64_t last_n = 0;       // tuples of the last scan
+    bool have_scan = false;
+    bool finished = false;
+    int scans_done = 0;
+    std::vector<uint64_t> chunk_ids;  // first read id of each chunk (C+1)
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+namespace mp {
+
+#define MP_HIP(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return ctx->fail(MP_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+#define MP_TRY(x)              \
+    do {                       \
+        int r_ = (x);          \
+        if (r_) return r_;     \
+    } while (0)
+
+// Grow-only device allocation charged against the ctx budget.
+int ensure(mp_ctx *ctx, DevArray &a, uint64_t bytes);
+void release(mp_ctx *ctx, DevArray &a);
+
+// Exclusive scan of n u32 or u64 values into out (n+1 entries: out[n] = total).
+int scan_u32(mp_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+int scan_u64(mp_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n);
+// Fill with a 32-bit pattern via a kernel (safe under graph capture).
+int fill_u32(mp_ctx *ctx, void *p, uint32_t v, uint64_t n_words);
+
+// stages (each .hip file)
+int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_first_id, int C);
+int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out);
+// Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:
+// 0 = adj byte, 1..8 = key byte p-1.  *cur selects the buffer holding the data.
+int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
+                       uint32_t pass_mask, int *passes_out);
+int iota_u32(mp_ctx *ctx, uint32_t *p, uint64_t n);
+int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st);
+int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st);
+int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
+                     uint64_t *bytes_out);
+
+}  // namespace mp

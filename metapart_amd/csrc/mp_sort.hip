@@ -1,0 +1,295 @@
+// Stable LSD radix sort of the scan's tuples by (k-mer u64, adj u8).
+//
+// The reference sorts 14-byte tuples with 9 stable byte passes per thread
+// segment: byte 8 (adj) first, then k-mer bytes 0..7 (sortKmerList and
+// radixPass, metapart.c:1494-1598, 2167-2198), after a range partition by
+// segment.  Segments are contiguous bucket ranges of the k-mer's top bits, so
+// one global stable sort gives the same array.  Stability keeps generation
+// order among equal (k-mer, adj), as the reference's counting passes do.
+//
+// MI355X design: structure-of-arrays tuples (keys u64, labels u32, adj u8;
+// 13 bytes), one histogram read for every pass up front, then one "onesweep"
+// kernel per 8-bit digit:
+//   - 512 lanes take a 4096-tuple tile (tile ids from an atomic counter, so
+//     earlier tiles are always resident or done),
+//   - each wave ranks its 512 tuples stably with 8 ballots per item
+//     (bit-sliced digit match) and per-wave digit counters in LDS,
+//   - per-digit tile counts are published and the digit's global prefix is
+//     found by decoupled look-back over earlier tiles (epoch-tagged status
+//     words, so the status array is never cleared between passes),
+//   - tuples are reordered by digit in LDS and written out, so consecutive
+//     lanes write consecutive addresses of each digit's run.
+// Passes whose digit is constant over the scan (e.g. the top byte when K<29)
+// are skipped: a stable pass on a constant digit is the identity.
+#include "mp_internal.h"
+
+namespace mp {
+
+constexpr int kSB = 512;
+constexpr int kSW = kSB / kWave;
+constexpr int kSI = 8;
+constexpr int kTile = kSB * kSI;
+constexpr int kRadix = 256;
+constexpr int kPasses = 9;
+
+constexpr uint64_t kStAgg = 1, kStInc = 2;
+constexpr uint64_t kStVal = (1ull << 46) - 1;
+
+__device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint64_t v)
+{
+    return ((uint64_t)epoch << 48) | (flag << 46) | v;
+}
+
+struct SortPass {
+    const uint64_t *keys_in;
+    const uint32_t *lab_in;
+    const uint8_t *adj_in;
+    uint64_t *keys_out;
+    uint32_t *lab_out;
+    uint8_t *adj_out;
+    uint64_t n;
+    int shift;  // < 0: the digit is the adj byte
+    const uint64_t *digit_base;
+    uint64_t *status;
+    uint32_t *tile_ctr;
+    uint32_t epoch;
+};
+
+__global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__restrict__ keys,
+                                                           const uint8_t *__restrict__ adj, uint64_t n,
+                                                           unsigned long long *__restrict__ ghist)
+{
+    __shared__ uint32_t h[kPasses][kRadix];
+    for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) (&h[0][0])[j] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        atomicAdd(&h[0][adj[i]], 1u);
+#pragma unroll
+        for (int b = 0; b < 8; b++) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) {
+        const uint32_t v = (&h[0][0])[j];
+        if (v) atomicAdd(&ghist[j], (unsigned long long)v);
+    }
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
+{
+    return shift < 0 ? (uint32_t)a : (uint32_t)((k >> shift) & 0xFF);
+}
+
+__global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
+{
+    __shared__ uint64_t s_keys[kTile];
+    __shared__ uint32_t s_lab[kTile];
+    __shared__ uint8_t s_adj[kTile];
+    __shared__ uint16_t s_cnt[kSW][kRadix];
+    __shared__ uint32_t s_dstart[kRadix];
+    __shared__ uint64_t s_gbase[kRadix];
+    __shared__ uint32_t s_wsum[kRadix / kWave];
+    __shared__ uint32_t s_tile;
+
+    const int tid = threadIdx.x;
+    const int wave = tid / kWave;
+    const unsigned lane = __lane_id();
+    for (int j = tid; j < kSW * kRadix; j += kSB) (&s_cnt[0][0])[j] = 0;
+    if (tid == 0) s_tile = atomicAdd(P.tile_ctr, 1u);
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t base = tile * kTile;
+
+    uint64_t k[kSI];
+    uint32_t l[kSI];
+    uint8_t a[kSI];
+    uint32_t d[kSI];
+    uint32_t rank[kSI];
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+        if (idx < P.n) {
+            k[i] = P.keys_in[idx];
+            l[i] = P.lab_in[idx];
+            a[i] = P.adj_in[idx];
+            d[i] = digit_of(k[i], a[i], P.shift);
+        } else {
+            k[i] = 0;
+            l[i] = 0;
+            a[i] = 0;
+            d[i] = kRadix;  // invalid
+        }
+    }
+    // stable rank within the wave: item-major, lane-minor order
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        const bool valid = d[i] < (uint32_t)kRadix;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d[i] >> b) & 1;
+            const uint64_t m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+        }
+        rank[i] = 0;
+        if (valid) {
+            const uint32_t below = (uint32_t)__popcll(peers & lt);
+            const uint32_t cnt = (uint32_t)__popcll(peers);
+            const uint32_t old = s_cnt[wave][d[i]];
+            rank[i] = old + below;
+            __builtin_amdgcn_wave_barrier();
+            if (below == cnt - 1) s_cnt[wave][d[i]] = (uint16_t)(old + cnt);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: exclusive offsets over waves, tile total, tile-local digit start
+    uint32_t total = 0;
+    if (tid < kRadix) {
+#pragma unroll
+        for (int w = 0; w < kSW; w++) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = (uint16_t)total;
+            total += c;
+        }
+        uint32_t inc = total;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(inc, o, kWave);
+            if (lane >= (unsigned)o) inc += v;
+        }
+        if (lane == kWave - 1) s_wsum[wave] = inc;
+        s_dstart[tid] = inc - total;  // wave-local exclusive prefix; waves added below
+    }
+    __syncthreads();
+    if (tid < kRadix) {
+        uint32_t woff = 0;
+        for (int w = 0; w < wave; w++) woff += s_wsum[w];
+        s_dstart[tid] += woff;
+        // decoupled look-back for digit tid
+        uint64_t prefix = 0;
+        uint64_t *st = P.status + tile * kRadix + tid;
+        if (tile == 0) {
+            st_agent(st, st_pack(P.epoch, kStInc, total));
+        } else {
+            st_agent(st, st_pack(P.epoch, kStAgg, total));
+            int64_t j = (int64_t)tile - 1;
+            while (j >= 0) {
+                const uint64_t s = ld_agent(P.status + (uint64_t)j * kRadix + tid);
+                const uint64_t flag = (s >> 46) & 3;
+                if ((uint32_t)(s >> 48) != P.epoch || flag == 0) continue;
+                prefix += s & kStVal;
+                if (flag == kStInc) break;
+                j--;
+            }
+            st_agent(st, st_pack(P.epoch, kStInc, prefix + total));
+        }
+        s_gbase[tid] = P.digit_base[tid] + prefix;
+    }
+    __syncthreads();
+    // reorder by digit in LDS
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        if (d[i] < (uint32_t)kRadix) {
+            const uint32_t lp = s_dstart[d[i]] + s_cnt[wave][d[i]] + rank[i];
+            s_keys[lp] = k[i];
+            s_lab[lp] = l[i];
+            s_adj[lp] = a[i];
+        }
+    }
+    __syncthreads();
+    const uint64_t nvalid = P.n - base < (uint64_t)kTile ? P.n - base : (uint64_t)kTile;
+    for (uint32_t j = tid; j < nvalid; j += kSB) {
+        const uint64_t kk = s_keys[j];
+        const uint8_t aa = s_adj[j];
+        const uint32_t dd = digit_of(kk, aa, P.shift);
+        const uint64_t g = s_gbase[dd] + (j - s_dstart[dd]);
+        P.keys_out[g] = kk;
+        P.lab_out[g] = s_lab[j];
+        P.adj_out[g] = aa;
+    }
+}
+
+int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
+                       uint32_t pass_mask, int *passes_out)
+{
+    *passes_out = 0;
+    if (n <= 1) return 0;
+    // histograms of every digit
+    MP_TRY(ensure(ctx, ctx->hist, sizeof(uint64_t) * (kPasses * kRadix + 64)));
+    unsigned long long *gh = (unsigned long long *)ctx->hist.p;
+    uint32_t *tile_ctr = (uint32_t *)(gh + kPasses * kRadix);
+    MP_TRY(fill_u32(ctx, gh, 0, 2 * kPasses * kRadix + 2 * 32));
+    {
+        uint64_t blocks = (n + kRadix * 16 - 1) / (kRadix * 16);
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)blocks), dim3(kRadix), 0, ctx->stream,
+                           (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p, n, gh);
+        MP_HIP(hipGetLastError());
+    }
+    uint64_t h[kPasses * kRadix];
+    MP_HIP(hipMemcpyAsync(h, gh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    // digit base offsets of the passes that do work
+    uint64_t base[kPasses][kRadix];
+    bool run[kPasses];
+    for (int p = 0; p < kPasses; p++) {
+        uint64_t acc = 0, mx = 0;
+        for (int r = 0; r < kRadix; r++) {
+            base[p][r] = acc;
+            acc += h[p * kRadix + r];
+            if (h[p * kRadix + r] > mx) mx = h[p * kRadix + r];
+        }
+        if (acc != n)
+            return ctx->fail(MP_EDEVICE, "sort histogram total %llu != %llu", (unsigned long long)acc,
+                             (unsigned long long)n);
+        run[p] = ((pass_mask >> p) & 1) && mx != n;
+    }
+    const uint64_t tiles = (n + kTile - 1) / kTile;
+    const uint64_t st_bytes = tiles * kRadix * sizeof(uint64_t);
+    if (ctx->status.bytes < st_bytes) {
+        MP_TRY(ensure(ctx, ctx->status, st_bytes));
+        MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
+        ctx->epoch = 0;
+    }
+    uint64_t *d_base = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_base, sizeof(base), ctx->stream));
+    MP_HIP(hipMemcpyAsync(d_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
+    int executed = 0;
+    for (int p = 0; p < kPasses; p++) {
+        if (!run[p]) continue;
+        if (++ctx->epoch == 0) {  // wrapped: old status words could alias
+            MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
+            ctx->epoch = 1;
+        }
+        const int src = *cur, dst = 1 - *cur;
+        SortPass P;
+        P.keys_in = (const uint64_t *)keys[src].p;
+        P.lab_in = (const uint32_t *)lab[src].p;
+        P.adj_in = (const uint8_t *)adj[src].p;
+        P.keys_out = (uint64_t *)keys[dst].p;
+        P.lab_out = (uint32_t *)lab[dst].p;
+        P.adj_out = (uint8_t *)adj[dst].p;
+        P.n = n;
+        P.shift = p == 0 ? -1 : 8 * (p - 1);
+        P.digit_base = d_base + p * kRadix;
+        P.status = (uint64_t *)ctx->status.p;
+        P.tile_ctr = tile_ctr + executed;
+        P.epoch = ctx->epoch;
+        hipLaunchKernelGGL(onesweep_kernel, dim3((unsigned)tiles), dim3(kSB), 0, ctx->stream, P);
+        MP_HIP(hipGetLastError());
+        *cur = dst;
+        executed++;
+    }
+    MP_HIP(hipFreeAsync(d_base, ctx->stream));
+    *passes_out = executed;
+    return 0;
+}
+
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
+{
+    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FF, passes_out);
+}
+
+}  // namespace mp
