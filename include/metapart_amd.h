@@ -46,6 +46,14 @@ enum {
  * Replaces genFastqIdx main (genFastqIdx.c:142-376).  nPE counts file PAIRS. */
 int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const char *const *files, int nfiles);
 
+/* The same chunk boundaries for one in-memory FASTQ file (single-end, or
+ * interleaved pairs when interleaved != 0): chunk_ofs receives up to
+ * num_chunks+1 byte offsets (the last is nbytes), *n_chunks the chunk count
+ * (smaller than num_chunks when the reference's near-end-of-file rule stops
+ * early, genFastqIdx.c:219-225). */
+int mp_index_buffer(const char *buf, uint64_t nbytes, int num_chunks, int interleaved, uint64_t *chunk_ofs,
+                    int *n_chunks);
+
 /* Range planning with one MPI rank: scan[S+1] and thread[S*T+1] bucket
  * boundaries from the exclusive-prefix histogram hist[2^20+1].
  * Replaces metapart.c:3720-3791.  Returns MP_EPLAN when a walk would write
@@ -73,6 +81,8 @@ typedef struct {
     int quirk_segments;       /* segments whose first run joined tuple 0's label */
     int sort_passes;          /* radix passes executed */
     float ms_enumerate, ms_sort, ms_filter, ms_union; /* device time */
+    float ms_sort_pass_avg;   /* mean duration of one radix scatter pass */
+    float ms_sort_hist;       /* the digit-histogram read before the passes */
 } mp_scan_stats;
 
 typedef struct {
@@ -111,6 +121,17 @@ int mp_load_reads(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *
  * until the next mp_load_* or mp_ctx_destroy). */
 int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const uint64_t *chunk_ofs,
                          const uint64_t *chunk_first_id, int C);
+
+/* m-mer histograms of the reads already loaded (no upload, no re-parse):
+ * counts as in mp_count_chunks (may be NULL), hist the exclusive prefix over
+ * buckets, 2^20+1 u64 as the kmerHist file holds it (may be NULL),
+ * reads_per_chunk C values (may be NULL).
+ * Replaces genHistograms.c:161-344 for a run that stays on one device. */
+int mp_count_loaded(mp_ctx *ctx, uint32_t *counts, uint64_t *hist, uint64_t *reads_per_chunk, uint64_t *total_kmers);
+
+/* Read id of the first record of every chunk of the loaded run (C+1 values,
+ * the last is the read count): what the KmerIdx readId fields hold. */
+int mp_chunk_first_ids(mp_ctx *ctx, uint64_t *ids);
 
 /* One scan over buckets [lo, hi) split into T segments at seg_bounds[0..T]
  * (thread ranges of the plan): enumerate, sort, filter, union, compress.

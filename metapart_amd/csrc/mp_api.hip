@@ -32,6 +32,11 @@ int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
             delete ctx;
             return MP_EDEVICE;
         }
+    for (auto &e : ctx->pev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return MP_EDEVICE;
+        }
     size_t fre = 0, tot = 0;
     (void)hipMemGetInfo(&fre, &tot);
     ctx->budget = cfg->hbm_budget ? cfg->hbm_budget : (uint64_t)(fre * 0.9);
@@ -52,6 +57,8 @@ int mp_ctx_destroy(mp_ctx *ctx)
                         &ctx->iaux[0], &ctx->iaux[1]};
     for (DevArray *a : arrs) release(ctx, *a);
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->pev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -114,6 +121,43 @@ int mp_count_chunks(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t
     return 0;
 }
 
+int mp_count_loaded(mp_ctx *ctx, uint32_t *counts, uint64_t *hist, uint64_t *reads_per_chunk, uint64_t *total_kmers)
+{
+    if (!ctx) return MP_EINVAL;
+    if (ctx->N == 0) return ctx->fail(MP_ESTATE, "mp_count_loaded before mp_load_reads");
+    MP_HIP(hipSetDevice(ctx->device));
+    const int C = ctx->C;
+    const uint64_t words = (uint64_t)C * (MP_NBUCKETS + 2);
+    MP_TRY(ensure(ctx, ctx->scratch, words * sizeof(uint32_t) + 4096));
+    MP_TRY(ensure(ctx, ctx->hist, (2 * (uint64_t)MP_NBUCKETS + 2) * sizeof(uint64_t)));
+    uint32_t *d_counts = (uint32_t *)ctx->scratch.p;
+    MP_TRY(count_chunks_device(ctx, ctx->chunk_ids.data(), C, d_counts));
+    uint64_t *d_tot = (uint64_t *)ctx->hist.p, *d_pre = d_tot + MP_NBUCKETS;
+    MP_TRY(bucket_totals(ctx, d_counts, C, d_tot));
+    // the exclusive prefix needs scan scratch separate from the count table
+    std::vector<uint64_t> pre(MP_NBUCKETS + 1);
+    if (counts) {  // before the scan below reuses the scratch area for its status words
+        MP_HIP(hipMemcpyAsync(counts, d_counts, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    MP_TRY(scan_u64(ctx, d_tot, d_pre, MP_NBUCKETS));
+    MP_HIP(hipMemcpyAsync(pre.data(), d_pre, sizeof(uint64_t) * (MP_NBUCKETS + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (hist) memcpy(hist, pre.data(), sizeof(uint64_t) * (MP_NBUCKETS + 1));
+    if (total_kmers) *total_kmers = pre[MP_NBUCKETS];
+    if (reads_per_chunk)
+        for (int c = 0; c < C; c++) reads_per_chunk[c] = ctx->chunk_ids[c + 1] - ctx->chunk_ids[c];
+    return 0;
+}
+
+int mp_chunk_first_ids(mp_ctx *ctx, uint64_t *ids)
+{
+    if (!ctx || !ids) return MP_EINVAL;
+    if (ctx->N == 0) return ctx->fail(MP_ESTATE, "no reads loaded");
+    for (int c = 0; c <= ctx->C; c++) ids[c] = ctx->chunk_ids[c];
+    return 0;
+}
+
 int mp_load_reads(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs,
                   const uint64_t *chunk_first_id, int C)
 {
@@ -165,6 +209,8 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     MP_TRY(filter_union_scan(ctx, n, seg_bounds, T, &st));
     st.tuples = n;
     st.sort_passes = passes;
+    st.ms_sort_pass_avg = ctx->sort_pass_avg;
+    st.ms_sort_hist = ctx->sort_hist_ms;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     if (stats) *stats = st;

@@ -64,18 +64,17 @@ __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, in
     pos[t] = lo;
 }
 
-__global__ void filter_kernel(GParams G)
+// Evaluates the run starting at tuple i (if i starts a run).  Returns 0 (not
+// evaluated), 1 (evaluated) or 2 (evaluated and zeroed).
+__device__ int filter_run(const GParams &G, uint64_t i)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G.n) return;
     uint64_t s0, s1;
     seg_of(G, i, s0, s1);
     const uint64_t k = G.keys[i];
-    if (i != s0 && G.keys[i - 1] == k) return;  // not a run start
+    if (i != s0 && G.keys[i - 1] == k) return 0;  // not a run start
     uint64_t j = i + 1;
     while (j < s1 && G.keys[j] == k) j++;
-    if (j == s1) return;  // the last run of a segment is never evaluated
-    atomicAdd(&G.stats[0], 1ull);
+    if (j == s1) return 0;  // the last run of a segment is never evaluated
     uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint64_t t = i; t < j; t++) {
         const uint8_t a = G.adj[t];
@@ -97,9 +96,8 @@ __global__ void filter_kernel(GParams G)
             nOut++;
         }
     }
-    if (!(nIn > 1 || nOut > 1)) return;
+    if (!(nIn > 1 || nOut > 1)) return 1;
     if (nIn > 1 && nOut > 1) {  // both sides branch: the k-mer is zeroed
-        atomicAdd(&G.stats[1], 1ull);
         for (uint64_t t = i; t < j;) {
             const uint64_t w = t >> 5;
             const uint64_t e = ((w + 1) << 5) < j ? ((w + 1) << 5) : j;
@@ -108,7 +106,7 @@ __global__ void filter_kernel(GParams G)
             atomicOr(&G.zbits[w], m);
             t = e;
         }
-        return;
+        return 2;
     }
     // (float)smax / degree compared with the double 0.2 (metapart.c:2356-2359, 2419)
     const float deg = (float)(int)(j - i);
@@ -121,6 +119,32 @@ __global__ void filter_kernel(GParams G)
             G.bad[L] = 1;
             G.filtered[L] = 1;
         }
+    }
+    return 1;
+}
+
+// grid-stride over tuples; run statistics are summed per block, one atomic each
+__global__ __launch_bounds__(256) void filter_kernel(GParams G)
+{
+    __shared__ uint32_t s_ev[256 / kWave], s_z[256 / kWave];
+    uint32_t ev = 0, z = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
+        const int r = filter_run(G, i);
+        ev += r >= 1;
+        z += r == 2;
+    }
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        ev += __shfl_down(ev, d, kWave);
+        z += __shfl_down(z, d, kWave);
+    }
+    if (__lane_id() == 0) { s_ev[threadIdx.x / kWave] = ev; s_z[threadIdx.x / kWave] = z; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b = 0;
+        for (int w = 0; w < 256 / kWave; w++) { a += s_ev[w]; b += s_z[w]; }
+        if (a) atomicAdd(&G.stats[0], (unsigned long long)a);
+        if (b) atomicAdd(&G.stats[1], (unsigned long long)b);
     }
 }
 
@@ -179,24 +203,37 @@ __device__ __forceinline__ void uf_union(uint32_t *parent, uint32_t a, uint32_t 
     }
 }
 
-__global__ void union_kernel(GParams G)
+// bad[] bytes -> one bit per read, small enough to stay in L2 for the union's
+// random lookups (20M reads: 2.5 MB)
+__global__ void bad_bits_kernel(const uint8_t *__restrict__ bad, uint64_t N, uint64_t *__restrict__ bits)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G.n) return;
-    uint64_t s0, s1;
-    seg_of(G, i, s0, s1);
-    const uint64_t k = G.keys[i];
-    if (i != s0 && G.keys[i - 1] == k) return;
-    if (k == 0) return;  // zeroed run or the all-A k-mer (metapart.c:2574)
-    uint64_t j = i + 1;
-    while (j < s1 && G.keys[j] == k) j++;
-    uint64_t c = i;
-    while (c < j && G.bad[G.lab[c]]) c++;
-    if (c >= j) return;
-    const uint32_t a = G.lab[c];
-    for (uint64_t t = c + 1; t < j; t++) {
-        const uint32_t b = G.lab[t];
-        if (b != a && !G.bad[b]) uf_union(G.parent, a, b);
+    const uint64_t m = __ballot(i < N && bad[i] != 0);
+    if (__lane_id() == 0 && i < N) bits[i >> 6] = m;
+}
+
+__device__ __forceinline__ bool is_bad(const uint64_t *bits, uint32_t x) { return (bits[x >> 6] >> (x & 63)) & 1; }
+
+__global__ void union_kernel(GParams G, const uint64_t *__restrict__ badbits)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
+        uint64_t s0, s1;
+        seg_of(G, i, s0, s1);
+        const uint64_t k = G.keys[i];
+        if (i != s0 && G.keys[i - 1] == k) continue;
+        if (k == 0) continue;  // zeroed run or the all-A k-mer (metapart.c:2574)
+        if (i + 1 >= s1 || G.keys[i + 1] != k) continue;  // one tuple: nothing to join
+        uint64_t j = i + 1;
+        while (j < s1 && G.keys[j] == k) j++;
+        uint64_t c = i;
+        while (c < j && is_bad(badbits, G.lab[c])) c++;
+        if (c >= j) continue;
+        const uint32_t a = G.lab[c];
+        for (uint64_t t = c + 1; t < j; t++) {
+            const uint32_t b = G.lab[t];
+            if (b != a && !is_bad(badbits, b)) uf_union(G.parent, a, b);
+        }
     }
 }
 
@@ -247,9 +284,25 @@ __global__ void root_weight_kernel(const uint32_t *__restrict__ parent, const ui
         s = root ? sizes[i] : 0;
         w[i] = s;
     }
+    // block totals: number of roots and the largest size, one atomic each
+    __shared__ uint32_t s_n[256 / kWave], s_m[256 / kWave];
     const uint64_t m = __ballot(root);
-    if (__lane_id() == 0 && m) atomicAdd(&st[0], (unsigned long long)__popcll(m));
-    if (root) atomicMax(&st[1], (unsigned long long)s);
+    uint32_t mx = s;
+    for (int d = kWave / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_down(mx, d, kWave));
+    if (__lane_id() == 0) {
+        s_n[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+        s_m[threadIdx.x / kWave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = 0, b = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) {
+            a += s_n[w];
+            b = max(b, s_m[w]);
+        }
+        if (a) atomicAdd(&st[0], (unsigned long long)a);
+        if (b) atomicMax(&st[1], (unsigned long long)b);
+    }
 }
 
 __global__ void plan_kernel(const uint64_t *__restrict__ excl, uint64_t N, uint64_t *__restrict__ W)
@@ -311,7 +364,8 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
     G.stats = d_stats;
     hipEventRecord(ctx->ev[2], ctx->stream);
     if (n) {
-        hipLaunchKernelGGL(filter_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, G);
+        const uint64_t fb = std::min<uint64_t>(nblk(n), 256 * 16);
+        hipLaunchKernelGGL(filter_kernel, dim3((unsigned)fb), dim3(256), 0, ctx->stream, G);
         MP_HIP(hipGetLastError());
         hipLaunchKernelGGL(apply_zero_kernel, dim3(nblk(nwords)), dim3(256), 0, ctx->stream, keys,
                            (const uint32_t *)ctx->zbits.p, nwords);
@@ -354,7 +408,13 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
                 MP_HIP(hipGetLastError());
             }
         }
-        hipLaunchKernelGGL(union_kernel, dim3(nblk(n)), dim3(256), 0, ctx->stream, G);
+        MP_TRY(ensure(ctx, ctx->scratch, ((N + 63) / 64 + 1) * sizeof(uint64_t)));
+        uint64_t *badbits = (uint64_t *)ctx->scratch.p;
+        hipLaunchKernelGGL(bad_bits_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint8_t *)G.bad, N,
+                           badbits);
+        MP_HIP(hipGetLastError());
+        const uint64_t ub = std::min<uint64_t>(nblk(n), 256 * 16);
+        hipLaunchKernelGGL(union_kernel, dim3((unsigned)ub), dim3(256), 0, ctx->stream, G, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.parent, (const uint8_t *)G.bad, N);

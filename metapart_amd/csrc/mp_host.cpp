@@ -6,7 +6,9 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -32,29 +34,21 @@ bool locate(uint64_t pos, const std::vector<uint64_t> &fsize, int numPE, int32_t
     return true;
 }
 
-}  // namespace
 
-extern "C" int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const char *const *files, int nfiles)
+using Probe = std::function<size_t(int file, uint64_t off, uint8_t *dst, size_t n)>;
+using Sink = std::function<int(const mp_fileindex_hdr &)>;
+
+// genFastqIdx.c:142-376 for single-end and interleaved files
+int index_core(int num_chunks, int numPE, int numSE, const std::vector<uint64_t> &fsize, const Probe &probe,
+               const Sink &sink)
 {
-    (void)nIL;
-    if (num_chunks <= 0 || nfiles <= 0 || !files) return MP_EINVAL;
-    if (nPE > 0) return MP_EINVAL;  // two-file paired-end input is not supported
-    const int numPE = 2 * nPE, numSE = nSE;
-    std::vector<uint64_t> fsize(nfiles);
+    const int nfiles = (int)fsize.size();
     uint64_t total = 0;
-    for (int i = 0; i < nfiles; i++) {
-        struct stat st;
-        if (stat(files[i], &st) != 0) return MP_EIO;
-        fsize[i] = (uint64_t)st.st_size;
-        total += fsize[i];
-    }
+    for (uint64_t v : fsize) total += v;
     const uint64_t per = total / (uint64_t)num_chunks;
-    std::string path = std::string(files[0]) + ".idx";
-    FILE *out = fopen(path.c_str(), "wb");
-    if (!out) return MP_EIO;
     mp_fileindex_hdr hdr;
     memset(&hdr, 0, sizeof(hdr));
-    std::vector<uint8_t> probe(5000 + 64);
+    std::vector<uint8_t> probe_buf(5000 + 64);
     int rc = 0, interleaved = 0;
     int64_t maxRead = 300;  // genFastqIdx.c:144, tightened after every chunk (303)
     for (int i = 0; i < num_chunks && !rc; i++) {
@@ -68,17 +62,13 @@ extern "C" int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const c
         if (i == 0) {
             hdr.offset2 = 0;
             hdr.readId = 0;
-            rc = mp_fileindex_write(out, &hdr, nullptr);
+            rc = sink(hdr);
             continue;
         }
-        FILE *fp = fopen(files[hdr.fileNo], "rb");
-        if (!fp) { rc = MP_EIO; break; }
-        fseek(fp, (long)hdr.offset, SEEK_SET);
-        std::fill(probe.begin(), probe.end(), 0);
-        const size_t nread = fread(probe.data(), 1, 3000, fp);
-        fclose(fp);
+        std::fill(probe_buf.begin(), probe_buf.end(), 0);
+        const size_t nread = probe(hdr.fileNo, hdr.offset, probe_buf.data(), 3000);
         if (!(nread > (uint64_t)maxRead)) { rc = MP_EINVAL; break; }
-        const uint8_t *b = probe.data();
+        const uint8_t *b = probe_buf.data();
         const size_t lim = 4990;
         bool found = false;
         size_t j = 0;
@@ -122,17 +112,69 @@ extern "C" int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const c
         hdr.offset += j;
         hdr.offset2 = 0;
         hdr.readId = 0;
-        rc = mp_fileindex_write(out, &hdr, nullptr);
+        rc = sink(hdr);
     }
     if (!rc) {  // sentinel: end of the last file (338-361)
         hdr.fileNo = nfiles - 1;
         hdr.offset = fsize[nfiles - 1];
         hdr.offset2 = 0;
         hdr.readId = 0;
-        rc = mp_fileindex_write(out, &hdr, nullptr);
+        rc = sink(hdr);
     }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const char *const *files, int nfiles)
+{
+    (void)nIL;
+    if (num_chunks <= 0 || nfiles <= 0 || !files) return MP_EINVAL;
+    if (nPE > 0) return MP_EINVAL;  // two-file paired-end input is not supported
+    std::vector<uint64_t> fsize(nfiles);
+    for (int i = 0; i < nfiles; i++) {
+        struct stat st;
+        if (stat(files[i], &st) != 0) return MP_EIO;
+        fsize[i] = (uint64_t)st.st_size;
+    }
+    std::string path = std::string(files[0]) + ".idx";
+    FILE *out = fopen(path.c_str(), "wb");
+    if (!out) return MP_EIO;
+    Probe probe = [&](int file, uint64_t off, uint8_t *dst, size_t n) -> size_t {
+        FILE *fp = fopen(files[file], "rb");
+        if (!fp) return 0;
+        fseek(fp, (long)off, SEEK_SET);
+        const size_t got = fread(dst, 1, n, fp);
+        fclose(fp);
+        return got;
+    };
+    Sink sink = [&](const mp_fileindex_hdr &h) -> int { return mp_fileindex_write(out, &h, nullptr); };
+    int rc = index_core(num_chunks, 0, nSE, fsize, probe, sink);
     if (fclose(out) != 0 && !rc) rc = MP_EIO;
     return rc;
+}
+
+extern "C" int mp_index_buffer(const char *buf, uint64_t nbytes, int num_chunks, int interleaved, uint64_t *chunk_ofs,
+                               int *n_chunks)
+{
+    if (!buf || !chunk_ofs || !n_chunks || num_chunks <= 0) return MP_EINVAL;
+    std::vector<uint64_t> fsize(1, nbytes);
+    Probe probe = [&](int, uint64_t off, uint8_t *dst, size_t n) -> size_t {
+        if (off >= nbytes) return 0;
+        const size_t got = (size_t)std::min<uint64_t>(n, nbytes - off);
+        memcpy(dst, buf + off, got);
+        return got;
+    };
+    int count = 0;
+    Sink sink = [&](const mp_fileindex_hdr &h) -> int {
+        if (count > num_chunks) return MP_EINVAL;
+        chunk_ofs[count++] = h.offset;
+        return 0;
+    };
+    const int rc = index_core(num_chunks, 0, interleaved ? 0 : 1, fsize, probe, sink);
+    if (rc) return rc;
+    *n_chunks = count - 1;
+    return 0;
 }
 
 // metapart.c:3720-3791 with one MPI rank: the process ranges equal the scan

@@ -57,6 +57,8 @@ struct mp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8]{};
+    hipEvent_t pev[24]{};  // per radix pass (timing of the dominant kernel)
+    float sort_pass_avg = 0, sort_hist_ms = 0;
     std::string err;
     uint64_t budget = 0, held = 0;
     uint16_t epoch = 0;  // look-back status epoch of the radix sort
@@ -124,6 +126,8 @@ int fill_u32(mp_ctx *ctx, void *p, uint32_t v, uint64_t n_words);
 // stages (each .hip file)
 int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_first_id, int C);
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
+// per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
+int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
 int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out);
 // Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:

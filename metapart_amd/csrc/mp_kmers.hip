@@ -351,6 +351,22 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
     });
 }
 
+__global__ void bucket_total_kernel(const uint32_t *__restrict__ counts, int C, uint64_t *__restrict__ tot)
+{
+    const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= MP_NBUCKETS) return;
+    uint64_t s = 0;
+    for (int c = 0; c < C; c++) s += counts[(size_t)c * (MP_NBUCKETS + 2) + b];
+    tot[b] = s;
+}
+
+int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot)
+{
+    hipLaunchKernelGGL(bucket_total_kernel, dim3(MP_NBUCKETS / 256), dim3(256), 0, ctx->stream, d_counts, C, d_tot);
+    MP_HIP(hipGetLastError());
+    return 0;
+}
+
 static KParams base_params(mp_ctx *ctx)
 {
     KParams P{};

@@ -21,14 +21,12 @@
 //     lanes write consecutive addresses of each digit's run.
 // Passes whose digit is constant over the scan (e.g. the top byte when K<29)
 // are skipped: a stable pass on a constant digit is the identity.
+#include <cstdlib>
+
 #include "mp_internal.h"
 
 namespace mp {
 
-constexpr int kSB = 512;
-constexpr int kSW = kSB / kWave;
-constexpr int kSI = 8;
-constexpr int kTile = kSB * kSI;
 constexpr int kRadix = 256;
 constexpr int kPasses = 9;
 
@@ -81,13 +79,18 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
     return shift < 0 ? (uint32_t)a : (uint32_t)((k >> shift) & 0xFF);
 }
 
+template <int kSB, int kSI>
 __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
 {
-    __shared__ uint64_t s_keys[kTile];
-    __shared__ uint32_t s_lab[kTile];
-    __shared__ uint8_t s_adj[kTile];
+    constexpr int kSW = kSB / kWave;
+    constexpr int kTile = kSB * kSI;
+    // one staging buffer (keys, then labels + adj) keeps LDS under half a CU,
+    // so two tiles run per CU and one tile's memory phases overlap the other's
+    // ranking and look-back
+    __shared__ uint64_t s_buf[kTile];
+    __shared__ uint8_t s_dig[kTile];
     __shared__ uint16_t s_cnt[kSW][kRadix];
-    __shared__ uint32_t s_dstart[kRadix];
+    __shared__ uint32_t s_dstart[kRadix];  // tile histogram first, then digit starts
     __shared__ uint64_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / kWave];
     __shared__ uint32_t s_tile;
@@ -96,6 +99,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     const int wave = tid / kWave;
     const unsigned lane = __lane_id();
     for (int j = tid; j < kSW * kRadix; j += kSB) (&s_cnt[0][0])[j] = 0;
+    if (tid < kRadix) s_dstart[tid] = 0;
     if (tid == 0) s_tile = atomicAdd(P.tile_ctr, 1u);
     __syncthreads();
     const uint64_t tile = s_tile;
@@ -121,6 +125,18 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
             d[i] = kRadix;  // invalid
         }
     }
+#pragma unroll
+    for (int i = 0; i < kSI; i++)
+        if (d[i] < (uint32_t)kRadix) atomicAdd(&s_dstart[d[i]], 1u);
+    __syncthreads();
+    // publish this tile's digit counts before ranking, so later tiles can look
+    // back past it while it is still busy
+    uint32_t total = 0;
+    if (tid < kRadix) {
+        total = s_dstart[tid];
+        uint64_t *st = P.status + tile * kRadix + tid;
+        st_agent(st, st_pack(P.epoch, tile == 0 ? kStInc : kStAgg, total));
+    }
     // stable rank within the wave: item-major, lane-minor order
     const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -145,14 +161,14 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // per digit: exclusive offsets over waves, tile total, tile-local digit start
-    uint32_t total = 0;
+    // per digit: exclusive offsets over waves and the tile-local digit start
     if (tid < kRadix) {
+        uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kSW; w++) {
             const uint32_t c = s_cnt[w][tid];
-            s_cnt[w][tid] = (uint16_t)total;
-            total += c;
+            s_cnt[w][tid] = (uint16_t)run;
+            run += c;
         }
         uint32_t inc = total;
         for (int o = 1; o < kWave; o <<= 1) {
@@ -169,11 +185,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
         s_dstart[tid] += woff;
         // decoupled look-back for digit tid
         uint64_t prefix = 0;
-        uint64_t *st = P.status + tile * kRadix + tid;
-        if (tile == 0) {
-            st_agent(st, st_pack(P.epoch, kStInc, total));
-        } else {
-            st_agent(st, st_pack(P.epoch, kStAgg, total));
+        if (tile != 0) {
             int64_t j = (int64_t)tile - 1;
             while (j >= 0) {
                 const uint64_t s = ld_agent(P.status + (uint64_t)j * kRadix + tid);
@@ -183,31 +195,42 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
                 if (flag == kStInc) break;
                 j--;
             }
-            st_agent(st, st_pack(P.epoch, kStInc, prefix + total));
+            st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, kStInc, prefix + total));
         }
         s_gbase[tid] = P.digit_base[tid] + prefix;
     }
     __syncthreads();
-    // reorder by digit in LDS
+    // tile-local position of every item, then two staged write-outs
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
         if (d[i] < (uint32_t)kRadix) {
-            const uint32_t lp = s_dstart[d[i]] + s_cnt[wave][d[i]] + rank[i];
-            s_keys[lp] = k[i];
-            s_lab[lp] = l[i];
-            s_adj[lp] = a[i];
+            rank[i] += s_dstart[d[i]] + s_cnt[wave][d[i]];
+            s_buf[rank[i]] = k[i];
+            s_dig[rank[i]] = (uint8_t)d[i];
         }
     }
     __syncthreads();
-    const uint64_t nvalid = P.n - base < (uint64_t)kTile ? P.n - base : (uint64_t)kTile;
+    const uint32_t nvalid = (uint32_t)(P.n - base < (uint64_t)kTile ? P.n - base : (uint64_t)kTile);
     for (uint32_t j = tid; j < nvalid; j += kSB) {
-        const uint64_t kk = s_keys[j];
-        const uint8_t aa = s_adj[j];
-        const uint32_t dd = digit_of(kk, aa, P.shift);
+        const uint32_t dd = s_dig[j];
+        P.keys_out[s_gbase[dd] + (j - s_dstart[dd])] = s_buf[j];
+    }
+    __syncthreads();
+    uint32_t *s_lab = (uint32_t *)s_buf;
+    uint8_t *s_adj = (uint8_t *)(s_lab + kTile);
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        if (d[i] < (uint32_t)kRadix) {
+            s_lab[rank[i]] = l[i];
+            s_adj[rank[i]] = a[i];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < nvalid; j += kSB) {
+        const uint32_t dd = s_dig[j];
         const uint64_t g = s_gbase[dd] + (j - s_dstart[dd]);
-        P.keys_out[g] = kk;
         P.lab_out[g] = s_lab[j];
-        P.adj_out[g] = aa;
+        P.adj_out[g] = s_adj[j];
     }
 }
 
@@ -221,6 +244,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     unsigned long long *gh = (unsigned long long *)ctx->hist.p;
     uint32_t *tile_ctr = (uint32_t *)(gh + kPasses * kRadix);
     MP_TRY(fill_u32(ctx, gh, 0, 2 * kPasses * kRadix + 2 * 32));
+    hipEventRecord(ctx->pev[0], ctx->stream);
     {
         uint64_t blocks = (n + kRadix * 16 - 1) / (kRadix * 16);
         if (blocks > 2048) blocks = 2048;
@@ -228,6 +252,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
                            (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p, n, gh);
         MP_HIP(hipGetLastError());
     }
+    hipEventRecord(ctx->pev[1], ctx->stream);
     uint64_t h[kPasses * kRadix];
     MP_HIP(hipMemcpyAsync(h, gh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
@@ -246,6 +271,19 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
                              (unsigned long long)n);
         run[p] = ((pass_mask >> p) & 1) && mx != n;
     }
+    // tile shape: lanes x items.  Default from a sweep on MI355X at 2.4e9
+    // tuples: 1024 x 8 (27.8 ms/pass) < 512 x 8 (32.8) < 512 x 16 (34.3)
+    // < 256 x 16 (39.5) ~ 256 x 8 (39.7).
+    int shape = ctx->cfg.sort_tile;
+    if (const char *e = getenv("MP_SORT_TILE")) shape = atoi(e);
+    int SB = 1024, SI = 6;
+    switch (shape) {
+    case 4096: SB = 512; SI = 8; break;
+    case 6144: SB = 512; SI = 12; break;
+    case 8192: SB = 1024; SI = 8; break;
+    default: SB = 1024; SI = 6; break;  // 6144 tuples on 1024 lanes
+    }
+    const uint64_t kTile = (uint64_t)SB * SI;
     const uint64_t tiles = (n + kTile - 1) / kTile;
     const uint64_t st_bytes = tiles * kRadix * sizeof(uint64_t);
     if (ctx->status.bytes < st_bytes) {
@@ -277,12 +315,29 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.status = (uint64_t *)ctx->status.p;
         P.tile_ctr = tile_ctr + executed;
         P.epoch = ctx->epoch;
-        hipLaunchKernelGGL(onesweep_kernel, dim3((unsigned)tiles), dim3(kSB), 0, ctx->stream, P);
+        hipEventRecord(ctx->pev[2 + 2 * executed], ctx->stream);
+        if (SB == 512 && SI == 8)
+            hipLaunchKernelGGL((onesweep_kernel<512, 8>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P);
+        else if (SB == 512 && SI == 12)
+            hipLaunchKernelGGL((onesweep_kernel<512, 12>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P);
+        else if (SB == 1024 && SI == 8)
+            hipLaunchKernelGGL((onesweep_kernel<1024, 8>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P);
+        else
+            hipLaunchKernelGGL((onesweep_kernel<1024, 6>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P);
         MP_HIP(hipGetLastError());
+        hipEventRecord(ctx->pev[3 + 2 * executed], ctx->stream);
         *cur = dst;
         executed++;
     }
     MP_HIP(hipFreeAsync(d_base, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    float tot = 0, ms = 0;
+    for (int e = 0; e < executed; e++) {
+        hipEventElapsedTime(&ms, ctx->pev[2 + 2 * e], ctx->pev[3 + 2 * e]);
+        tot += ms;
+    }
+    ctx->sort_pass_avg = executed ? tot / executed : 0.f;
+    hipEventElapsedTime(&ctx->sort_hist_ms, ctx->pev[0], ctx->pev[1]);
     *passes_out = executed;
     return 0;
 }
