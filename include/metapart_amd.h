@@ -148,8 +148,14 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
  * Replaces metapart.c:4010-4056 and unionFindPeReads (2801-2867). */
 int mp_finish(mp_ctx *ctx, uint64_t *W, mp_finish_stats *stats);
 
-/* Final parent label of every read (read count entries). */
+/* Parent label of every read (read count entries): the component root after
+ * the last scan (a BAD read is its own parent here, where the reference
+ * stores the sentinel 0xBADBADBAD), or the final root after mp_finish. */
 int mp_get_parent(mp_ctx *ctx, uint64_t *parent);
+
+/* BAD flag of every read (1 = marked by the frequency filter and not yet
+ * reset by the pair step).  Debug and parity only. */
+int mp_get_bad(mp_ctx *ctx, uint8_t *bad);
 
 /* The last scan's sorted, filtered tuple array in the reference's 14-byte
  * layout (enumerateKmers.c:352-365: k-mer u64, adj u8, id/UINT32_MAX u8,

@@ -85,6 +85,7 @@ def lib():
             "mp_scan": (i, [vp, i, u64, u64, P64, i, u64, ct.POINTER(ScanStats)]),
             "mp_finish": (i, [vp, P64, ct.POINTER(FinishStats)]),
             "mp_get_parent": (i, [vp, P64]),
+            "mp_get_bad": (i, [vp, vp]),
             "mp_dump_tuples": (i, [vp, vp, u64, P64]),
             "mp_write_partitions": (i, [vp, cp, i, P64, ct.POINTER(ct.c_int64), P64]),
         }
@@ -99,7 +100,7 @@ def lib():
 # exported symbol list (what include/metapart_amd.h declares)
 SYMBOLS = ["mp_index_fastq", "mp_index_buffer", "mp_plan_ranges", "mp_ctx_create", "mp_ctx_destroy",
            "mp_last_error", "mp_ctx_device_bytes", "mp_count_chunks", "mp_count_loaded", "mp_chunk_first_ids",
-           "mp_load_reads", "mp_load_reads_device", "mp_scan", "mp_finish", "mp_get_parent", "mp_dump_tuples",
+           "mp_load_reads", "mp_load_reads_device", "mp_scan", "mp_finish", "mp_get_parent", "mp_get_bad", "mp_dump_tuples",
            "mp_write_partitions"]
 
 
@@ -235,6 +236,11 @@ class Context:
     def parent(self) -> np.ndarray:
         out = np.zeros(self.N, dtype=np.uint64)
         self._check(lib().mp_get_parent(self._h, _p(out)), "mp_get_parent")
+        return out
+
+    def bad(self) -> np.ndarray:
+        out = np.zeros(self.N, dtype=np.uint8)
+        self._check(lib().mp_get_bad(self._h, out.ctypes.data), "mp_get_bad")
         return out
 
     def dump_tuples(self) -> np.ndarray:

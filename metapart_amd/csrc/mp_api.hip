@@ -245,6 +245,16 @@ int mp_get_parent(mp_ctx *ctx, uint64_t *parent)
     return 0;
 }
 
+int mp_get_bad(mp_ctx *ctx, uint8_t *bad)
+{
+    if (!ctx || !bad) return MP_EINVAL;
+    if (ctx->N == 0) return ctx->fail(MP_ESTATE, "no reads loaded");
+    MP_HIP(hipSetDevice(ctx->device));
+    MP_HIP(hipMemcpyAsync(bad, ctx->bad.p, ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n)
 {
     if (!ctx || !n) return MP_EINVAL;

@@ -1,0 +1,130 @@
+"""Regenerate tests/golden/golden.json: SHA-256 of every file the REFERENCE
+implementation (built from its sources by oracle/Makefile into oracle/_ref)
+writes for a set of seeded inputs.  Run here, where the reference sources
+exist; the JSON is committed and is what tests compare the oracle and the GPU
+tools against.
+
+    python tests/golden/make_golden.py
+
+Inputs are either the reference's own fixture (tests/golden/reads.fastq, a
+copy of data/reads.fastq: 2,500 interleaved 90 bp records) or FASTQ made by
+the seeded generator (metapart_amd/bin/mp_synth) with the listed arguments.
+The .idx and KmerIdx files are hashed with the bytes the reference leaves
+uninitialised (record padding, readIdExtra of record 0) set to zero.
+"""
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+SYNTH = os.path.join(ROOT, "metapart_amd", "bin", "mp_synth")
+REC = 4194344
+
+# name: (input, C, K, S, T, mode, variant)
+SYNTH_INPUTS = {
+    "tiny": ["--preset", "0"],
+    "c1": ["--preset", "1"],
+    "small150": ["--preset", "0", "--reads", "20000", "--genomes", "6", "--len", "150", "--err", "0.01", "--seed", "5"],
+    "var": ["--preset", "0", "--reads", "20000", "--len-min", "60", "--len-max", "160", "--nrate", "0.002",
+            "--err", "0.01", "--seed", "11"],
+    "il150": ["--preset", "0", "--reads", "20000", "--genomes", "6", "--len", "150", "--err", "0.01", "--seed", "12",
+              "--interleaved"],
+    "gc_rep": ["--preset", "0", "--reads", "40000", "--genomes", "10", "--glen-min", "20000", "--glen-max", "60000",
+               "--gc-min", "0.3", "--gc-max", "0.7", "--repeats", "3", "--len", "150", "--err", "0.01", "--seed", "21"],
+}
+CASES = [
+    ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
+    ("sample_T1", "reads.fastq", 4, 27, 1, 1, "il", ""),
+    ("sample_S2T2", "reads.fastq", 4, 27, 2, 2, "il", ""),
+    ("sample_min3", "reads.fastq", 4, 27, 1, 1, "il", "min"),
+    ("tiny", "tiny", 4, 27, 1, 1, "se", ""),
+    ("c1_T1", "c1", 8, 27, 1, 1, "se", ""),
+    ("c1_T2", "c1", 8, 27, 1, 2, "se", ""),
+    ("c1_S3T2_k31", "c1", 8, 31, 3, 2, "se", ""),
+    ("c1_min3", "c1", 8, 27, 1, 1, "se", "min"),
+    ("small_k32", "small150", 4, 32, 1, 1, "se", ""),
+    ("small_k31_S2T2", "small150", 8, 31, 2, 2, "se", ""),
+    ("var_k21_S2T3", "var", 6, 21, 2, 3, "se", ""),
+    ("var_min3_T2", "var", 4, 31, 1, 2, "se", "min"),
+    ("il_k31_T2", "il150", 8, 31, 1, 2, "il", ""),
+    ("il_min3", "il150", 4, 27, 1, 1, "il", "min"),
+    ("gcrep_k31", "gc_rep", 16, 31, 1, 1, "se", ""),
+    ("gcrep_k31_S4T2", "gc_rep", 16, 31, 4, 2, "se", ""),
+]
+
+
+def masked_sha(path):
+    b = bytearray(open(path, "rb").read())
+    for r in range(len(b) // REC):
+        o = r * REC
+        if r == 0:
+            b[o + 16] = 0
+        b[o + 17:o + 20] = b"\0\0\0"
+        b[o + REC - 4:o + REC] = b"\0\0\0\0"
+    return hashlib.sha256(bytes(b)).hexdigest()
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def make_input(name, dst):
+    if name.endswith(".fastq"):
+        shutil.copy(os.path.join(HERE, name), dst)
+    else:
+        subprocess.run([SYNTH] + SYNTH_INPUTS[name] + ["-o", dst], check=True)
+
+
+def hash_outputs(d, K, variant):
+    out = {}
+    for f in sorted(os.listdir(d)):
+        if f.startswith("in.fastq."):
+            p = os.path.join(d, f)
+            out[f[len("in.fastq."):]] = masked_sha(p) if ("idx" in f.lower()) else sha(p)
+    od = os.path.join(d, "out", "Rank0")
+    for f in sorted(os.listdir(od)):
+        out["out/" + f] = sha(os.path.join(od, f))
+    return out
+
+
+def run_reference(case, work):
+    name, inp, C, K, S, T, mode, variant = case
+    d = os.path.join(work, name)
+    os.makedirs(os.path.join(d, "out", "Rank0"))
+    make_input(inp, os.path.join(d, "in.fastq"))
+    a = ["0", "0", "1"] if mode == "il" else ["0", "1", "0"]
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    q = dict(cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+    subprocess.run([os.path.join(REF, "genFastqIdx"), str(C)] + a + ["in.fastq"], **q)
+    subprocess.run([os.path.join(REF, "genHistograms_metapart" + variant), str(K)] + a + ["in.fastq"], **q)
+    subprocess.run([os.path.join(REF, "metapart" + variant), "-o", "./out", "2000000000", str(K), str(S), str(T)] + a +
+                   ["in.fastq"], **q)
+    return hash_outputs(d, K, variant)
+
+
+def main():
+    if not os.path.exists(os.path.join(REF, "metapart")):
+        sys.exit("build the reference first: make -C oracle all")
+    work = tempfile.mkdtemp(prefix="golden_")
+    try:
+        res = {"inputs": SYNTH_INPUTS, "cases": []}
+        for case in CASES:
+            h = run_reference(case, work)
+            name, inp, C, K, S, T, mode, variant = case
+            res["cases"].append({"name": name, "input": inp, "C": C, "K": K, "S": S, "T": T, "mode": mode,
+                                 "variant": variant, "sha256": h})
+            print(name, len(h), "files")
+        with open(os.path.join(HERE, "golden.json"), "w") as f:
+            json.dump(res, f, indent=1, sort_keys=True)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

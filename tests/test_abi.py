@@ -1,0 +1,86 @@
+"""The C ABI library without a GPU: it loads, exports every entry point the
+header declares, and its host-side parts (chunk indexer, range planner) match
+the reference (golden hashes) and the oracle."""
+import ctypes as ct
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from _helpers import BIN, ROOT, golden, make_input, masked_sha, oracle
+
+import metapart_amd as mp
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "metapart_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mp_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ct.CDLL(mp.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+    assert sorted(mp.SYMBOLS) == names
+
+
+def test_ctx_create_without_gpu_fails_cleanly():
+    # on a machine without a GPU the library reports an error instead of aborting
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(mp.MPError):
+        mp.Context(k=31)
+
+
+@pytest.mark.parametrize("case", [c for c in golden()["cases"]], ids=[c["name"] for c in golden()["cases"]])
+def test_index_tool_matches_reference(case, tmp_path):
+    d = str(tmp_path)
+    make_input(case["input"], os.path.join(d, "in.fastq"))
+    a = ["0", "0", "1"] if case["mode"] == "il" else ["0", "1", "0"]
+    subprocess.run([os.path.join(BIN, "genFastqIdx"), str(case["C"])] + a + ["in.fastq"], cwd=d, check=True,
+                   stdout=subprocess.DEVNULL)
+    assert masked_sha(os.path.join(d, "in.fastq.idx")) == case["sha256"]["idx"]
+
+
+def test_index_buffer_equals_index_file(tmp_path):
+    buf = mp.synth(1)
+    p = os.path.join(str(tmp_path), "in.fastq")
+    open(p, "wb").write(buf.tobytes())
+    subprocess.run([os.path.join(BIN, "genFastqIdx"), "8", "0", "1", "0", "in.fastq"], cwd=str(tmp_path), check=True,
+                   stdout=subprocess.DEVNULL)
+    raw = np.fromfile(p + ".idx", dtype=np.uint8).reshape(-1, mp.FILEINDEX_BYTES)
+    offs = raw[:, 0:8].copy().view(np.uint64).ravel()
+    assert np.array_equal(mp.index_buffer(buf, 8), offs)
+
+
+def test_plan_ranges_matches_oracle():
+    rng = np.random.default_rng(11)
+    L = oracle()
+    for trial in range(6):
+        counts = rng.poisson(rng.uniform(0.5, 40), size=2 ** 20).astype(np.uint64)
+        H = np.zeros(2 ** 20 + 1, dtype=np.uint64)
+        H[1:] = np.cumsum(counts)
+        for S, T in [(1, 1), (2, 2), (4, 3), (7, 1)]:
+            scan = np.zeros(S + 1, dtype=np.uint64)
+            thr = np.zeros(S * T + 1, dtype=np.uint64)
+            rc = L.orc_plan_ranges(H.ctypes.data, S, T, scan.ctypes.data, thr.ctypes.data)
+            assert rc == 0
+            s2, t2 = mp.plan_ranges(H, S, T)
+            assert np.array_equal(scan, s2) and np.array_equal(thr, t2)
+
+
+def test_plan_ranges_rejects_degenerate():
+    H = np.zeros(2 ** 20 + 1, dtype=np.uint64)
+    H[1:] = 5  # every k-mer in bucket 0: no thread split is possible
+    with pytest.raises(mp.MPError) as e:
+        mp.plan_ranges(H, 1, 4)
+    assert e.value.code == -3

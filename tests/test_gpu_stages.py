@@ -1,0 +1,60 @@
+"""Stage-by-stage parity of the HIP path (through the C ABI) with the oracle on
+the same seeded inputs: per-chunk m-mer counts, the sorted and filtered tuple
+array of a scan (reference 14-byte layout), BAD flags and parent labels after
+that scan, and the final parent labels after the pair step."""
+import os
+
+import numpy as np
+import pytest
+
+from _helpers import BAD, oracle_run, read_kmeridx
+
+import metapart_amd as mp
+
+CASES = [
+    # name, synth overrides, C, K, S, T, min3, interleaved, dump_scan
+    ("plain_k27", dict(n_reads=6000, seed=41), 4, 27, 1, 1, False, False, 0),
+    ("plain_k31_T2", dict(n_reads=6000, seed=42, read_len=150), 4, 31, 1, 2, False, False, 0),
+    ("plain_k31_S3_scan2", dict(n_reads=8000, seed=43, read_len=150), 6, 31, 3, 2, False, False, 2),
+    ("plain_k20_S2_scan1", dict(n_reads=8000, seed=44, n_rate=0.003), 4, 20, 2, 1, False, False, 1),
+    ("min3_k27", dict(n_reads=8000, seed=45), 4, 27, 1, 1, True, False, 0),
+    ("il_k31", dict(n_reads=8000, seed=46, read_len=150, interleaved=1), 4, 31, 1, 1, False, True, 0),
+    ("varlen_k25_T3", dict(n_reads=9000, seed=47, read_len_min=40, read_len_max=180), 6, 25, 1, 3, False, False, 0),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_stages_match_oracle(case, tmp_path):
+    name, over, C, K, S, T, min3, il, dump = case
+    print("gpu stage case", name, flush=True)
+    buf = mp.synth(0, **over)
+    p = os.path.join(str(tmp_path), "in.fastq")
+    open(p, "wb").write(buf.tobytes())
+    ref = oracle_run(p, C, K, S, T, interleaved=il, min3=min3, dump_scan=dump)
+    _, ref_counts = read_kmeridx(p + (".KmerIdx_3min%d" % K if min3 else ".KmerIdx%d" % K))
+
+    with mp.Context(k=K, min3=min3) as ctx:
+        cofs = mp.index_buffer(buf, C, interleaved=il)
+        ctx.load(buf, cofs)
+        hist, reads, tot, counts = ctx.count(want_counts=True)
+        assert np.array_equal(counts, ref_counts[:-1])
+        assert int(hist[-1]) == tot == int(ref_counts.sum())
+        scan, thread = mp.plan_ranges(hist, S, T)
+        for s in range(S):
+            lo, hi = int(scan[s]), int(scan[s + 1])
+            st = ctx.scan(s, lo, hi, thread[s * T:(s + 1) * T + 1], int(hist[min(hi, mp.NBUCKETS)] - hist[lo]))
+            if s == dump:
+                tup = ctx.dump_tuples()
+                assert tup.shape == ref["tuples"].shape
+                assert np.array_equal(tup, ref["tuples"])
+                par = ctx.parent().astype(np.uint64)
+                bad = ctx.bad().astype(bool)
+                ref_par = ref["parent_scan"]
+                ref_bad = ref_par == BAD
+                assert np.array_equal(bad, ref_bad)
+                assert np.array_equal(par[~bad], ref_par[~ref_bad])
+                assert np.array_equal(par[bad], np.nonzero(bad)[0].astype(np.uint64))
+        W, fs = ctx.finish()
+        assert np.array_equal(ctx.parent(), ref["parent"])
+        assert W[0] == 0 and W[16] == ctx.N and np.all(np.diff(W.astype(np.int64)) >= 0)
