@@ -51,7 +51,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->scanst, &ctx->hpart, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -134,12 +134,8 @@ int mp_count_loaded(mp_ctx *ctx, uint32_t *counts, uint64_t *hist, uint64_t *rea
     MP_TRY(count_chunks_device(ctx, ctx->chunk_ids.data(), C, d_counts));
     uint64_t *d_tot = (uint64_t *)ctx->hist.p, *d_pre = d_tot + MP_NBUCKETS;
     MP_TRY(bucket_totals(ctx, d_counts, C, d_tot));
-    // the exclusive prefix needs scan scratch separate from the count table
     std::vector<uint64_t> pre(MP_NBUCKETS + 1);
-    if (counts) {  // before the scan below reuses the scratch area for its status words
-        MP_HIP(hipMemcpyAsync(counts, d_counts, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        MP_HIP(hipStreamSynchronize(ctx->stream));
-    }
+    if (counts) MP_HIP(hipMemcpyAsync(counts, d_counts, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     MP_TRY(scan_u64(ctx, d_tot, d_pre, MP_NBUCKETS));
     MP_HIP(hipMemcpyAsync(pre.data(), d_pre, sizeof(uint64_t) * (MP_NBUCKETS + 1), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));

@@ -21,6 +21,7 @@
 #include "mp_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 namespace mp {
@@ -320,6 +321,82 @@ __global__ void plan_kernel(const uint64_t *__restrict__ excl, uint64_t N, uint6
     W[p] = lo;
 }
 
+// ---- pair step (unionFindPeReads, metapart.c:2810-2860) on compressed labels:
+// parent[x] is the root of x's component, sizes[r] its read count.
+__device__ __forceinline__ bool pair_candidate(const uint32_t *parent, const uint32_t *sizes, uint64_t j, uint32_t &a,
+                                               uint32_t &b)
+{
+    a = parent[2 * j];
+    b = parent[2 * j + 1];
+    return a != b && (sizes[a] <= MP_PAIR_GATE || sizes[b] <= MP_PAIR_GATE);
+}
+
+// components of the candidate-pair graph, rooted at their largest id
+__global__ void pair_graph_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
+                                  uint32_t *gparent, uint64_t P)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t a, b;
+    if (j < P && pair_candidate(parent, sizes, j, a, b)) uf_union(gparent, a, b);
+}
+
+__global__ void compress_all_kernel(uint32_t *parent, uint64_t N)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t x = (uint32_t)i;
+    for (;;) {
+        const uint32_t p = ld_agent(&parent[x]);
+        if (p == x) break;
+        x = p;
+    }
+    __hip_atomic_store(&parent[i], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 1 for a candidate pair inside a candidate-graph component of > 1000 reads
+__global__ void pair_classify_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
+                                     const uint32_t *__restrict__ gparent, const uint32_t *__restrict__ gsize,
+                                     uint32_t *__restrict__ ordered, uint64_t P)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    uint32_t a, b;
+    ordered[j] = pair_candidate(parent, sizes, j, a, b) && gsize[gparent[a]] > MP_PAIR_GATE;
+}
+
+__global__ void pair_list_kernel(const uint32_t *__restrict__ ordered, const uint64_t *__restrict__ off, uint64_t P,
+                                 uint64_t *__restrict__ idx)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < P && ordered[j]) idx[off[j]] = j;
+}
+
+// Reads of candidate-graph components of <= 1000 reads take the component's
+// largest id; every other initial root of such a component is one union.
+__global__ void pair_apply_kernel(uint32_t *__restrict__ parent, const uint32_t *__restrict__ gparent,
+                                  const uint32_t *__restrict__ gsize, uint64_t N, unsigned long long *__restrict__ unions)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool u = false;
+    if (i < N) {
+        const uint32_t g = gparent[i];
+        if (gsize[g] <= MP_PAIR_GATE) {
+            u = parent[i] == (uint32_t)i && g != (uint32_t)i;
+            parent[i] = g;
+        }
+    }
+    const uint64_t m = __ballot(u);
+    if (__lane_id() == 0 && m) atomicAdd(unions, (unsigned long long)__popcll(m));
+}
+
+// Reads of the larger components take the labels of the ordered host pass.
+__global__ void pair_merge_kernel(uint32_t *__restrict__ parent, const uint32_t *__restrict__ host_parent,
+                                  const uint32_t *__restrict__ gparent, const uint32_t *__restrict__ gsize, uint64_t N)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N && gsize[gparent[i]] > MP_PAIR_GATE) parent[i] = host_parent[i];
+}
+
 static inline unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st)
@@ -439,7 +516,8 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
 }
 
 // ---------------------------------------------------------------- finish
-// Sequential pair step (unionFindPeReads, metapart.c:2801-2867) on the host.
+// Sequential pair step (unionFindPeReads, metapart.c:2801-2867) on the host,
+// for the pairs whose outcome depends on the order.
 static uint32_t hfind(uint32_t *parent, uint32_t x)
 {
     while (parent[x] != x) {
@@ -461,22 +539,84 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     MP_TRY(fill_u32(ctx, sizes, 0, N));
     hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent, sizes, N);
     MP_HIP(hipGetLastError());
-    std::vector<uint32_t> hp(N), hs(N);
-    MP_HIP(hipMemcpyAsync(hp.data(), parent, N * 4, hipMemcpyDeviceToHost, ctx->stream));
-    MP_HIP(hipMemcpyAsync(hs.data(), sizes, N * 4, hipMemcpyDeviceToHost, ctx->stream));
-    MP_HIP(hipStreamSynchronize(ctx->stream));
-    uint64_t unions = 0, cand = 0;
-    for (uint64_t i = 0; i + 1 < N; i += 2) {
-        const uint32_t a = hfind(hp.data(), (uint32_t)i), b = hfind(hp.data(), (uint32_t)(i + 1));
-        if (a == b) continue;
-        if (hs[a] <= MP_PAIR_GATE || hs[b] <= MP_PAIR_GATE) {
-            cand++;
-            unions++;
-            if (a > b) { hp[b] = a; hs[a] += hs[b]; hs[b] = 0; }
-            else { hp[a] = b; hs[b] += hs[a]; hs[a] = 0; }
+    // Pair step (unionFindPeReads): pair (2j, 2j+1) joins its two components
+    // unless both hold more than 1000 reads at that moment.  Sizes only grow,
+    // so a pair whose components are both > 1000 now never joins, and the
+    // other pairs ("candidates") form a graph G over the current components.
+    // A component of G holding <= 1000 reads in total can never contain a
+    // component > 1000, so every candidate inside it joins whatever the order:
+    // it ends as one component rooted at its largest id.  Only candidates
+    // inside larger components of G depend on the order; those run the exact
+    // sequential loop on the host, on their own (they share no reads with the
+    // rest).
+    const uint64_t P = N / 2;
+    uint64_t *d_small = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_small, 64 * sizeof(uint64_t), ctx->stream));
+    MP_TRY(fill_u32(ctx, d_small, 0, 128));
+    unsigned long long *d_unions = (unsigned long long *)(d_small + 40);
+    MP_TRY(ensure(ctx, ctx->gparent, N * sizeof(uint32_t)));
+    MP_TRY(ensure(ctx, ctx->gsize, N * sizeof(uint32_t)));
+    MP_TRY(ensure(ctx, ctx->wg_cnt, (P + 1) * sizeof(uint32_t)));
+    MP_TRY(ensure(ctx, ctx->wg_off, (P + 2) * sizeof(uint64_t)));
+    uint32_t *gparent = (uint32_t *)ctx->gparent.p, *gsize = (uint32_t *)ctx->gsize.p;
+    uint32_t *ordered = (uint32_t *)ctx->wg_cnt.p;
+    uint64_t n_ordered = 0;
+    std::vector<uint32_t> hp, hs;
+    std::vector<uint64_t> hidx;
+    if (P) {
+        MP_HIP(hipMemcpyAsync(gparent, parent, N * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(pair_graph_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+                           (const uint32_t *)sizes, gparent, P);
+        hipLaunchKernelGGL(compress_all_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, gparent, N);
+        MP_TRY(fill_u32(ctx, gsize, 0, N));
+        hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)gparent, gsize,
+                           N);
+        hipLaunchKernelGGL(pair_classify_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+                           (const uint32_t *)sizes, (const uint32_t *)gparent, (const uint32_t *)gsize, ordered, P);
+        MP_HIP(hipGetLastError());
+        MP_TRY(scan_u32(ctx, ordered, (uint64_t *)ctx->wg_off.p, P));
+        MP_HIP(hipMemcpyAsync(&n_ordered, (uint64_t *)ctx->wg_off.p + P, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        if (n_ordered) {  // the ordered pairs, and the labels and sizes they start from
+            MP_TRY(ensure(ctx, ctx->scratch, n_ordered * sizeof(uint64_t) + 64));
+            uint64_t *d_idx = (uint64_t *)ctx->scratch.p;
+            hipLaunchKernelGGL(pair_list_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)ordered,
+                               (const uint64_t *)ctx->wg_off.p, P, d_idx);
+            MP_HIP(hipGetLastError());
+            hp.resize(N);
+            hs.resize(N);
+            hidx.resize(n_ordered);
+            MP_HIP(hipMemcpyAsync(hidx.data(), d_idx, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(hp.data(), parent, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(hs.data(), sizes, N * 4, hipMemcpyDeviceToHost, ctx->stream));
         }
+        hipLaunchKernelGGL(pair_apply_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
+                           (const uint32_t *)gparent, (const uint32_t *)gsize, N, d_unions);
+        MP_HIP(hipGetLastError());
     }
-    MP_HIP(hipMemcpyAsync(parent, hp.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+    uint64_t unions = 0;
+    double host_ms = 0;
+    if (n_ordered) {
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint64_t q = 0; q < n_ordered; q++) {
+            const uint64_t i = 2 * hidx[q];
+            const uint32_t a = hfind(hp.data(), (uint32_t)i), b = hfind(hp.data(), (uint32_t)(i + 1));
+            if (a == b) continue;
+            if (hs[a] <= MP_PAIR_GATE || hs[b] <= MP_PAIR_GATE) {
+                unions++;
+                if (a > b) { hp[b] = a; hs[a] += hs[b]; hs[b] = 0; }
+                else { hp[a] = b; hs[b] += hs[a]; hs[a] = 0; }
+            }
+        }
+        host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        // sizes[] is recomputed below: it stages the host labels
+        MP_HIP(hipMemcpyAsync(sizes, hp.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(pair_merge_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
+                           (const uint32_t *)sizes, (const uint32_t *)gparent, (const uint32_t *)gsize, N);
+        MP_HIP(hipGetLastError());
+    }
     // every BAD read was reset to itself by the pair step
     MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
     hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, (const uint8_t *)ctx->bad.p, N);
@@ -486,9 +626,6 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     // partition plan over root ids in ascending order
     MP_TRY(ensure(ctx, ctx->wg_off, (N + 1) * sizeof(uint64_t)));
     MP_TRY(ensure(ctx, ctx->wg_cnt, N * sizeof(uint32_t)));
-    uint64_t *d_small = nullptr;
-    MP_HIP(hipMallocAsync((void **)&d_small, 64 * sizeof(uint64_t), ctx->stream));
-    MP_TRY(fill_u32(ctx, d_small, 0, 128));
     hipLaunchKernelGGL(root_weight_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
                        (const uint32_t *)sizes, (uint32_t *)ctx->wg_cnt.p, N, (unsigned long long *)(d_small + 32));
     MP_HIP(hipGetLastError());
@@ -502,13 +639,14 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     MP_HIP(hipStreamSynchronize(ctx->stream));
     for (int p = 0; p <= MP_NPART; p++) W[p] = res[p];
     if (st) {
-        st->pair_unions = unions;
-        st->pair_candidates = cand;
+        st->pair_unions = unions + res[40];
+        st->pair_candidates = n_ordered;
         st->components = res[32];
         st->largest = res[33];
         float ms = 0;
         hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]);
         st->ms_device = ms;
+        st->ms_host = (float)host_ms;
     }
     return 0;
 }

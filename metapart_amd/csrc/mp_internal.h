@@ -75,9 +75,12 @@ struct mp_ctx {
     mp::DevArray fq, rec_off, seq_off, seq_len, base_off, pos_off, packed, nmask;
     // graph state
     mp::DevArray parent, bad, filtered, sizes;
+    mp::DevArray gparent, gsize;  // pair step: components of the candidate-pair graph
     // scan buffers (ping-pong)
     mp::DevArray keys[2], lab[2], adj[2], zbits;
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
+    mp::DevArray scanst;  // look-back status of the exclusive scans
+    mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
     // writer
     mp::DevArray out, items[2], ilens[2], iaux[2];
     std::vector<uint64_t> chunk_ofs;  // byte offset of each chunk (C+1)

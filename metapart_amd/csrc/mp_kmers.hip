@@ -22,6 +22,9 @@
 // read, one lane per read, restating the reference loop on the raw bytes.
 #include "mp_internal.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace mp {
 
 constexpr int kRG = 64;       // reads per workgroup, plain kernels
@@ -186,6 +189,200 @@ __global__ __launch_bounds__(kKBlock) void kmer_plain_kernel(KParams P)
             P.wg_cnt[blockIdx.x] = t;
         }
     }
+}
+
+// ---------------------------------------------------------------- m-mer histogram
+// The per-chunk table has 2^20 bins per chunk, far beyond LDS, and one global
+// atomic per k-mer is bound by the atomic rate of the memory side.  So the
+// histogram is built in three LDS-privatised steps:
+//   1. tiles of up to 1024 reads of one chunk count their k-mers per digit
+//      (top 6 bits of the 20-bit bucket) in LDS,
+//   2. a scan over (chunk, digit, tile) places every tile's digit run, and the
+//      tiles write the low 14 bucket bits of each k-mer there (u16),
+//   3. each (chunk, digit) partition is counted in a 16K-bin LDS histogram and
+//      written to its slice of the chunk's row.
+// Bucket order inside a run does not matter to a histogram, so step 2 takes
+// slots with LDS atomics instead of a stable ranking.
+constexpr int kHDigitBits = 6;
+constexpr int kHDigits = 1 << kHDigitBits;
+constexpr int kHLowBits = 20 - kHDigitBits;
+constexpr int kHLow = 1 << kHLowBits;
+constexpr uint32_t kHReads = 1024;       // reads per tile (steps 1, 2)
+constexpr uint64_t kHSlab = 4u << 20;    // items per workgroup (step 3)
+
+struct HParams {
+    const uint64_t *chunk_ids;  // C+1 first read ids
+    const uint32_t *tile0;      // C+1 first tile of each chunk
+    int C;
+    uint32_t *tcnt;             // [c][d][tile of c] digit counts
+    const uint64_t *toff;       // exclusive scan of tcnt
+    uint16_t *items;            // low bucket bits, grouped by (chunk, digit)
+};
+
+struct HPiece {
+    uint64_t b, e;   // item range
+    uint32_t part;   // chunk * kHDigits + digit
+    uint32_t split;  // partition spread over several pieces: add atomically
+};
+
+template <int PHASE>  // 0: digit counts per tile, 1: scatter of the low bits
+__global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H)
+{
+    __shared__ uint32_t s_pref[kRG + 1];
+    __shared__ uint64_t s_base[kRG];
+    __shared__ uint32_t s_dig[kHDigits];
+    __shared__ uint64_t s_out[kHDigits];
+    const int tid = threadIdx.x;
+    const uint32_t t = blockIdx.x;
+    int c = 0, hi = H.C - 1;  // chunk: last c with tile0[c] <= t
+    while (c < hi) {
+        const int mid = (c + hi + 1) >> 1;
+        if (H.tile0[mid] <= t) c = mid; else hi = mid - 1;
+    }
+    const uint32_t tl = t - H.tile0[c], Tc = H.tile0[c + 1] - H.tile0[c];
+    const uint64_t rb = H.chunk_ids[c] + (uint64_t)tl * kHReads;
+    const uint64_t re = min(rb + kHReads, H.chunk_ids[c + 1]);
+    const uint64_t mbase = (uint64_t)kHDigits * H.tile0[c] + tl;
+    if (tid < kHDigits) {
+        s_dig[tid] = 0;
+        if (PHASE == 1) s_out[tid] = H.toff[mbase + (uint64_t)tid * Tc];
+    }
+    for (uint64_t r0 = rb; r0 < re; r0 += kRG) {
+        const int nr = (int)min((uint64_t)kRG, re - r0);
+        __syncthreads();  // previous group done with s_pref / s_base
+        const uint64_t p0 = P.pos_off[r0];
+        if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
+        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        __syncthreads();
+        const uint32_t total = s_pref[nr];
+        for (uint32_t q = tid; q < total; q += kKBlock) {
+            int lo = 0, h = nr - 1;
+            while (lo < h) {
+                const int mid = (lo + h + 1) >> 1;
+                if (s_pref[mid] <= q) lo = mid; else h = mid - 1;
+            }
+            const uint32_t p = q - s_pref[lo];
+            uint64_t canon;
+            bool fwd;
+            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            const uint32_t b = (uint32_t)(canon >> P.bshift);
+            const uint32_t d = b >> kHLowBits;
+            const uint32_t slot = atomicAdd(&s_dig[d], 1u);
+            if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
+        }
+    }
+    if (PHASE == 0) {
+        __syncthreads();
+        if (tid < kHDigits) H.tcnt[mbase + (uint64_t)tid * Tc] = s_dig[tid];
+    }
+}
+
+// start of every (chunk, digit) partition, and the end of the last
+__global__ void hist_bounds_kernel(HParams H, uint64_t *__restrict__ bounds)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > H.C * kHDigits) return;
+    if (i == H.C * kHDigits) { bounds[i] = H.toff[(uint64_t)kHDigits * H.tile0[H.C]]; return; }
+    const int c = i / kHDigits, d = i % kHDigits;
+    const uint32_t Tc = H.tile0[c + 1] - H.tile0[c];
+    bounds[i] = H.toff[(uint64_t)kHDigits * H.tile0[c] + (uint64_t)d * Tc];
+}
+
+__global__ __launch_bounds__(1024) void hist_count_kernel(const uint16_t *__restrict__ items,
+                                                          const HPiece *__restrict__ pieces,
+                                                          uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t s_h[kHLow];
+    const HPiece pc = pieces[blockIdx.x];
+    for (int i = threadIdx.x; i < kHLow; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    // 8 items per 16-byte load on the aligned middle, single items at the ends
+    const uint64_t a = min(pc.e, (pc.b + 7) & ~7ull), z = max(a, pc.e & ~7ull);
+    for (uint64_t i = pc.b + threadIdx.x; i < a; i += blockDim.x) atomicAdd(&s_h[items[i]], 1u);
+    for (uint64_t i = z + threadIdx.x; i < pc.e; i += blockDim.x) atomicAdd(&s_h[items[i]], 1u);
+    const uint4 *v = (const uint4 *)(items + a);
+    const uint64_t nv = (z - a) / 8;
+    for (uint64_t j = threadIdx.x; j < nv; j += blockDim.x) {
+        const uint4 w = v[j];
+        const uint32_t x[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            atomicAdd(&s_h[x[k] & 0xFFFF], 1u);
+            atomicAdd(&s_h[x[k] >> 16], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t c = pc.part / kHDigits, d = pc.part % kHDigits;
+    uint32_t *row = counts + (size_t)c * (MP_NBUCKETS + 2) + ((size_t)d << kHLowBits);
+    for (int i = threadIdx.x; i < kHLow; i += blockDim.x) {
+        const uint32_t h = s_h[i];
+        if (!pc.split) row[i] = h;
+        else if (h) atomicAdd(&row[i], h);
+    }
+}
+
+static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk_first_id, int C, uint32_t *d_counts)
+{
+    std::vector<uint32_t> tile0(C + 1);
+    tile0[0] = 0;
+    for (int c = 0; c < C; c++) {
+        const uint64_t nr = chunk_first_id[c + 1] - chunk_first_id[c];
+        tile0[c + 1] = tile0[c] + (uint32_t)((nr + kHReads - 1) / kHReads);
+    }
+    const uint64_t NT = tile0[C];
+    if (NT == 0) return 0;
+    const uint64_t M = (uint64_t)kHDigits * NT;
+    const uint64_t nparts = (uint64_t)C * kHDigits;
+    // hpart: chunk ids (C+1 u64) | tile0 (C+1 u32, padded) | bounds (nparts+1 u64) | tile counts (M u32)
+    const uint64_t o_ids = 0, o_t0 = o_ids + 8 * (C + 1), o_bd = o_t0 + 8 * ((C + 2) / 2),
+                   o_tc = o_bd + 8 * (nparts + 1), bytes = o_tc + 4 * M + 64;
+    MP_TRY(ensure(ctx, ctx->hpart, bytes));
+    MP_TRY(ensure(ctx, ctx->wg_off, (M + 1) * sizeof(uint64_t)));
+    uint8_t *hb = (uint8_t *)ctx->hpart.p;
+    HParams H{};
+    H.chunk_ids = (const uint64_t *)(hb + o_ids);
+    H.tile0 = (const uint32_t *)(hb + o_t0);
+    H.C = C;
+    H.tcnt = (uint32_t *)(hb + o_tc);
+    H.toff = (const uint64_t *)ctx->wg_off.p;
+    uint64_t *d_bounds = (uint64_t *)(hb + o_bd);
+    MP_HIP(hipMemcpyAsync(hb + o_ids, chunk_first_id, 8 * (C + 1), hipMemcpyHostToDevice, ctx->stream));
+    MP_HIP(hipMemcpyAsync(hb + o_t0, tile0.data(), 4 * (C + 1), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(hist_tile_kernel<0>, dim3((unsigned)NT), dim3(kKBlock), 0, ctx->stream, P, H);
+    MP_HIP(hipGetLastError());
+    MP_TRY(scan_u32(ctx, H.tcnt, (uint64_t *)ctx->wg_off.p, M));
+    hipLaunchKernelGGL(hist_bounds_kernel, dim3((unsigned)((nparts + 256) / 256)), dim3(256), 0, ctx->stream, H,
+                       d_bounds);
+    MP_HIP(hipGetLastError());
+    std::vector<uint64_t> bounds(nparts + 1);
+    MP_HIP(hipMemcpyAsync(bounds.data(), d_bounds, 8 * (nparts + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t total = bounds[nparts];
+    if (total == 0) return 0;
+    // the low bits stage in the second tuple buffer, unused until the first scan's sort
+    MP_TRY(ensure(ctx, ctx->keys[1], total * sizeof(uint16_t) + 64));
+    H.items = (uint16_t *)ctx->keys[1].p;
+    hipLaunchKernelGGL(hist_tile_kernel<1>, dim3((unsigned)NT), dim3(kKBlock), 0, ctx->stream, P, H);
+    MP_HIP(hipGetLastError());
+    std::vector<HPiece> pieces;
+    for (uint64_t p = 0; p < nparts; p++) {
+        const uint64_t b = bounds[p], e = bounds[p + 1];
+        if (e == b) continue;
+        const uint64_t n = (e - b + kHSlab - 1) / kHSlab;
+        for (uint64_t j = 0; j < n; j++)
+            pieces.push_back({b + j * kHSlab, std::min(e, b + (j + 1) * kHSlab), (uint32_t)p, n > 1 ? 1u : 0u});
+    }
+    HPiece *d_pieces = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_pieces, pieces.size() * sizeof(HPiece), ctx->stream));
+    MP_HIP(hipMemcpyAsync(d_pieces, pieces.data(), pieces.size() * sizeof(HPiece), hipMemcpyHostToDevice,
+                          ctx->stream));
+    hipLaunchKernelGGL(hist_count_kernel, dim3((unsigned)pieces.size()), dim3(1024), 0, ctx->stream,
+                       (const uint16_t *)H.items, (const HPiece *)d_pieces, d_counts);
+    MP_HIP(hipGetLastError());
+    MP_HIP(hipFreeAsync(d_pieces, ctx->stream));
+    // the host vectors above must outlive the copies
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 // ---------------------------------------------------------------- 3-minimizer
@@ -396,11 +593,10 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
     if (ctx->cfg.min3) {
         hipLaunchKernelGGL(kmer_min3_kernel<kModeHist>, dim3((unsigned)((ctx->N + kKBlock - 1) / kKBlock)),
                            dim3(kKBlock), 0, ctx->stream, P);
+        MP_HIP(hipGetLastError());
     } else {
-        hipLaunchKernelGGL(kmer_plain_kernel<kModeHist>, dim3((unsigned)((ctx->N + kRG - 1) / kRG)), dim3(kKBlock), 0,
-                           ctx->stream, P);
+        MP_TRY(count_plain_partitioned(ctx, P, chunk_first_id, C, d_counts));
     }
-    MP_HIP(hipGetLastError());
     MP_HIP(hipFreeAsync(d_ids, ctx->stream));
     return 0;
 }

@@ -272,8 +272,8 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         run[p] = ((pass_mask >> p) & 1) && mx != n;
     }
     // tile shape: lanes x items.  Default from a sweep on MI355X at 2.4e9
-    // tuples: 1024 x 8 (27.8 ms/pass) < 512 x 8 (32.8) < 512 x 16 (34.3)
-    // < 256 x 16 (39.5) ~ 256 x 8 (39.7).
+    // tuples: 1024 x 6 (two tiles per CU, 22.8 ms/pass) < 1024 x 8 (one
+    // tile per CU, 27.8) < 512 x 8 (32.8) < 512 x 16 (34.3).
     int shape = ctx->cfg.sort_tile;
     if (const char *e = getenv("MP_SORT_TILE")) shape = atoi(e);
     int SB = 1024, SI = 6;

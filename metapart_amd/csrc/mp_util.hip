@@ -154,8 +154,8 @@ static int scan_impl(mp_ctx *ctx, const T *in, uint64_t *out, uint64_t n)
         return fill_u32(ctx, out, 0, 2);
     }
     uint64_t tiles = (n + kScanTile - 1) / kScanTile;
-    MP_TRY(ensure(ctx, ctx->scratch, (tiles + 64) * sizeof(uint64_t)));
-    uint64_t *status = (uint64_t *)ctx->scratch.p;
+    MP_TRY(ensure(ctx, ctx->scanst, (tiles + 64) * sizeof(uint64_t)));
+    uint64_t *status = (uint64_t *)ctx->scanst.p;
     uint32_t *ctr = (uint32_t *)(status + tiles);
     MP_TRY(fill_u32(ctx, status, 0, tiles * 2 + 2));
     hipLaunchKernelGGL(scan_kernel<T>, dim3((unsigned)tiles), dim3(kScanBlock), 0, ctx->stream, in, out, n, status, ctr);

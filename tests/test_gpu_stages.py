@@ -20,7 +20,12 @@ CASES = [
     ("min3_k27", dict(n_reads=8000, seed=45), 4, 27, 1, 1, True, False, 0),
     ("il_k31", dict(n_reads=8000, seed=46, read_len=150, interleaved=1), 4, 31, 1, 1, False, True, 0),
     ("varlen_k25_T3", dict(n_reads=9000, seed=47, read_len_min=40, read_len_max=180), 6, 25, 1, 3, False, False, 0),
+    # 40 small genomes: components of a few hundred reads that the pair step
+    # chains past 1000 reads, so part of it runs in the reference's order
+    ("pairs_ordered", dict(n_reads=20000, n_genomes=40, genome_len_min=4000, genome_len_max=8000, abund_sigma=1.0,
+                           seed=8), 4, 31, 1, 1, False, False, 0),
 ]
+ORDERED = {"pairs_ordered"}
 
 
 @pytest.mark.gpu
@@ -57,4 +62,6 @@ def test_stages_match_oracle(case, tmp_path):
                 assert np.array_equal(par[bad], np.nonzero(bad)[0].astype(np.uint64))
         W, fs = ctx.finish()
         assert np.array_equal(ctx.parent(), ref["parent"])
+        if name in ORDERED:
+            assert fs.pair_candidates > 0
         assert W[0] == 0 and W[16] == ctx.N and np.all(np.diff(W.astype(np.int64)) >= 0)
