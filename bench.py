@@ -31,6 +31,19 @@ HBM_PEAK_GBS = 8000.0
 TUPLE_BYTES = 13  # u64 k-mer + u32 label + u8 adj
 
 
+def measured_traffic(reads: int):
+    """HBM bytes per radix pass from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (counter runs cannot share a process with the
+    timed run), or None when no summary exists for this read count."""
+    p = os.path.join(ROOT, "profiles", "onesweep_kernel_traffic_c2_%dM.md" % (reads // 1_000_000))
+    if reads % 1_000_000 or not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch = "):
+            return {"bytes": int(line.split("=")[1]), "source": os.path.relpath(p, ROOT)}
+    return None
+
+
 def cpu_baseline(sample_reads: int, K: int, cores: int = 1):
     """The reference implementation (built from its sources into oracle/_ref) on
     the first sample_reads reads of the workload, one thread: genFastqIdx +
@@ -145,6 +158,7 @@ def main():
     achieved = per_pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     sort_bytes = n * (8 + 2 * passes * TUPLE_BYTES)
     sort_ms = float(sc["ms_sort"])
+    traffic = measured_traffic(args.reads)
     out = {
         "metric": "reads partitioned/s at k=31 on 1 GPU (plain path); % HBM peak in sort",
         "value": plain["reads_per_s"],
@@ -162,7 +176,9 @@ def main():
                                "k=%d m=10, 16 chunks, S=1, T=1, plain" % (args.reads, args.k),
                    "reads": args.reads, "k": args.k, "chunks": args.chunks, "scans": 1, "segments": 1},
         "roofline": {"bound": "hbm", "kernel": "onesweep_kernel (one radix scatter pass)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "bytes_per_launch": per_pass_bytes, "launch_ms": pass_ms, "tuples": n},
         "sort": {"passes": passes, "digit_bits": 8, "tuple_bytes": TUPLE_BYTES, "algorithmic_bytes": sort_bytes,
                  "ms": sort_ms, "hbm_frac": sort_bytes / (sort_ms * 1e-3) / 1e12 / (HBM_PEAK_GBS / 1e3)
