@@ -116,9 +116,10 @@ __device__ int filter_run(const GParams &G, uint64_t i)
         const uint8_t a = G.adj[t];
         const int bi = a & 7, ai = (a >> 3) & 7;
         if ((nIn > 1 && bi != maxIn) || (nOut > 1 && ai != maxOut) || ratio) {
+            // filteredReads; the refilter step turns it into BAD (the label is a
+            // root here, so that step's parent reset changes nothing for it)
             const uint32_t L = G.lab[t];
-            G.bad[L] = 1;
-            G.filtered[L] = 1;
+            if (!G.filtered[L]) G.filtered[L] = 1;
         }
     }
     return 1;
@@ -165,17 +166,16 @@ __global__ void apply_zero_kernel(uint64_t *__restrict__ keys, const uint32_t *_
 __global__ void refilter_kernel(uint8_t *__restrict__ bad, const uint8_t *__restrict__ filtered,
                                 uint32_t *__restrict__ parent, uint64_t N, unsigned long long *__restrict__ nbad)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool b = false;
-    if (i < N) {
+    uint32_t cnt = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
         if (filtered[i] && !bad[i]) {
             bad[i] = 1;
             parent[i] = (uint32_t)i;
         }
-        b = bad[i] != 0;
+        cnt += bad[i] != 0;
     }
-    const uint64_t m = __ballot(b);
-    if (__lane_id() == 0 && m) atomicAdd(nbad, (unsigned long long)__popcll(m));
+    block_add(cnt, nbad);
 }
 
 __device__ __forceinline__ uint32_t uf_find(uint32_t *parent, uint32_t x)
@@ -376,17 +376,16 @@ __global__ void pair_list_kernel(const uint32_t *__restrict__ ordered, const uin
 __global__ void pair_apply_kernel(uint32_t *__restrict__ parent, const uint32_t *__restrict__ gparent,
                                   const uint32_t *__restrict__ gsize, uint64_t N, unsigned long long *__restrict__ unions)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool u = false;
-    if (i < N) {
+    uint32_t cnt = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
         const uint32_t g = gparent[i];
         if (gsize[g] <= MP_PAIR_GATE) {
-            u = parent[i] == (uint32_t)i && g != (uint32_t)i;
+            cnt += parent[i] == (uint32_t)i && g != (uint32_t)i;
             parent[i] = g;
         }
     }
-    const uint64_t m = __ballot(u);
-    if (__lane_id() == 0 && m) atomicAdd(unions, (unsigned long long)__popcll(m));
+    block_add(cnt, unions);
 }
 
 // Reads of the larger components take the labels of the ordered host pass.
@@ -448,7 +447,7 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
                            (const uint32_t *)ctx->zbits.p, nwords);
         MP_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(refilter_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.bad,
+    hipLaunchKernelGGL(refilter_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, G.bad,
                        (const uint8_t *)G.filtered, G.parent, N, d_stats + 2);
     MP_HIP(hipGetLastError());
     hipEventRecord(ctx->ev[3], ctx->stream);
@@ -591,7 +590,7 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
             MP_HIP(hipMemcpyAsync(hp.data(), parent, N * 4, hipMemcpyDeviceToHost, ctx->stream));
             MP_HIP(hipMemcpyAsync(hs.data(), sizes, N * 4, hipMemcpyDeviceToHost, ctx->stream));
         }
-        hipLaunchKernelGGL(pair_apply_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
+        hipLaunchKernelGGL(pair_apply_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, parent,
                            (const uint32_t *)gparent, (const uint32_t *)gsize, N, d_unions);
         MP_HIP(hipGetLastError());
     }

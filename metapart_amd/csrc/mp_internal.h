@@ -34,6 +34,27 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 
+// Adds v summed over the workgroup to *ctr with one atomic (blockDim.x a
+// multiple of 64, at most 1024; every lane of the workgroup must call it).
+__device__ __forceinline__ void block_add(uint32_t v, unsigned long long *ctr)
+{
+    __shared__ uint32_t s_part[16];
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_down(v, d, kWave);
+    if (__lane_id() == 0) s_part[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (unsigned w = 0; w < blockDim.x / kWave; w++) t += s_part[w];
+        if (t) atomicAdd(ctr, (unsigned long long)t);
+    }
+}
+// grid of at most 2048 workgroups of 256 lanes for a grid-stride loop over n items
+inline unsigned stride_grid(uint64_t n)
+{
+    const uint64_t b = (n + 255) / 256;
+    return (unsigned)(b < 2048 ? (b ? b : 1) : 2048);
+}
+
 // Reverse-complement of a 2-bit packed k-mer (first base most significant,
 // A0 C1 G2 T3, complement = 3 - code): reverse the 2-bit groups of ~x.
 __device__ __forceinline__ uint64_t revcomp(uint64_t x, int K)

@@ -196,14 +196,15 @@ __global__ __launch_bounds__(kKBlock) void kmer_plain_kernel(KParams P)
 // atomic per k-mer is bound by the atomic rate of the memory side.  So the
 // histogram is built in three LDS-privatised steps:
 //   1. tiles of up to 1024 reads of one chunk count their k-mers per digit
-//      (top 6 bits of the 20-bit bucket) in LDS,
+//      (top 5 bits of the 20-bit bucket) in LDS,
 //   2. a scan over (chunk, digit, tile) places every tile's digit run, and the
-//      tiles write the low 14 bucket bits of each k-mer there (u16),
-//   3. each (chunk, digit) partition is counted in a 16K-bin LDS histogram and
+//      tiles write the low 15 bucket bits of each k-mer there (u16); few
+//      digits keep few partially written lines open per tile,
+//   3. each (chunk, digit) partition is counted in a 32K-bin LDS histogram and
 //      written to its slice of the chunk's row.
 // Bucket order inside a run does not matter to a histogram, so step 2 takes
 // slots with LDS atomics instead of a stable ranking.
-constexpr int kHDigitBits = 6;
+constexpr int kHDigitBits = 5;
 constexpr int kHDigits = 1 << kHDigitBits;
 constexpr int kHLowBits = 20 - kHDigitBits;
 constexpr int kHLow = 1 << kHLowBits;

@@ -141,17 +141,34 @@ __global__ void rec_finalize_kernel(const uint8_t *__restrict__ fq, uint64_t n_r
 }
 
 // 64 bases per thread: two words of the 2-bit stream and one word of the N stream.
-__global__ void pack_kernel(const uint8_t *__restrict__ fq, const uint64_t *__restrict__ seq_off,
-                            const uint64_t *__restrict__ base_off, uint64_t n_rec, uint64_t total_bases,
-                            uint64_t n_threads, uint64_t *__restrict__ packed, uint64_t *__restrict__ nmask)
+__global__ __launch_bounds__(256) void pack_kernel(const uint8_t *__restrict__ fq, const uint64_t *__restrict__ seq_off,
+                                                   const uint64_t *__restrict__ base_off, uint64_t n_rec,
+                                                   uint64_t total_bases, uint64_t n_threads,
+                                                   uint64_t *__restrict__ packed, uint64_t *__restrict__ nmask)
 {
+    // reads overlapping this workgroup's 256 x 64 bases: one search over all
+    // reads by the first lanes, then every lane searches only that range
+    __shared__ uint64_t s_r[2];
+    const uint64_t wb = (uint64_t)blockIdx.x * blockDim.x * 64;
+    if (threadIdx.x < 2) {
+        const uint64_t target = threadIdx.x == 0 ? wb : min(wb + (uint64_t)blockDim.x * 64, total_bases) - 1;
+        uint64_t lo = 0, hi = n_rec;
+        if (wb < total_bases) {
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) >> 1;
+                if (base_off[mid] <= target) lo = mid; else hi = mid - 1;
+            }
+        }
+        s_r[threadIdx.x] = lo;
+    }
+    __syncthreads();
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_threads) return;
     const uint64_t b0 = t * 64;
     uint64_t w0 = 0, w1 = 0, nm = 0;
     if (b0 < total_bases) {
         // read containing base b0: last r with base_off[r] <= b0
-        uint64_t lo = 0, hi = n_rec;
+        uint64_t lo = s_r[0], hi = s_r[1];
         while (lo < hi) {
             uint64_t mid = (lo + hi + 1) >> 1;
             if (base_off[mid] <= b0) lo = mid; else hi = mid - 1;
