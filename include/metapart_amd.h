@@ -27,6 +27,7 @@ extern "C" {
 #define MP_NBUCKETS 1048576u /* 4^10 m-mer buckets, m = 10 (genHistograms.c:105) */
 #define MP_NPART 16          /* output partitions (metapart.c:4016) */
 #define MP_PAIR_GATE 1000u   /* size gate of the pair step (metapart.c:4024) */
+#define MP_MAX_SAMPLES 256   /* samples per batch (one 8-bit sort digit) */
 
 enum {
     MP_OK = 0,
@@ -171,6 +172,42 @@ int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n);
  * Replaces writeOutput3 (metapart.c:956-1066). */
 int mp_write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
                         uint64_t *bytes_out);
+
+/* ---------------------------------------------------------------- batches
+ * Batched form.  The reference partitions one sample per process (metapart.c
+ * main); these entry points have no counterpart there and serve callers with
+ * many independent samples.  n samples, each a complete single-end FASTQ with
+ * an even, non-zero record count, go through one launch sequence: one parse,
+ * one enumeration, one sort whose last pass groups the tuples by sample, one
+ * filter/union pass with one segment per sample, one pair step, and per-sample
+ * plans.  Every sample's labels, plan and output files are those of a plain
+ * run of that sample alone with one chunk, one scan and one segment
+ * (numThreads = 1): mp_load_reads with C = 1, mp_scan over [0, 2^20),
+ * mp_finish, mp_write_partitions with T = 1. */
+typedef struct {
+    uint64_t reads;               /* records of the sample */
+    uint64_t tuples;              /* k-mer tuples of the sample */
+    uint64_t components;          /* components after the pair step */
+    uint64_t largest;             /* largest component */
+    uint64_t W[MP_NPART + 1];     /* partition plan over the sample's own read ids */
+} mp_sample_result;
+
+/* Loads n (<= MP_MAX_SAMPLES) host FASTQ buffers back to back into the ctx. */
+int mp_batch_load(mp_ctx *ctx, int n, const char *const *fq, const uint64_t *nbytes);
+
+/* Same for samples already back to back in device memory: sample i holds
+ * bytes [sample_ofs[i], sample_ofs[i+1]) of d_fq (sample_ofs[n] = nbytes). */
+int mp_batch_load_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, int n, const uint64_t *sample_ofs);
+
+/* Partitions every loaded sample; res (n entries) may be NULL. */
+int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_stats, mp_finish_stats *finish_stats);
+
+/* Final labels of sample s over its own read ids (reads entries). */
+int mp_batch_get_parent(mp_ctx *ctx, int s, uint32_t *parent);
+
+/* Output files OUT/sample<s>/Rank0/FromTid0_ToRank<p>.paired.fastq of every
+ * sample (out_dir NULL: build the streams on the device only). */
+int mp_batch_write(mp_ctx *ctx, const char *out_dir, uint64_t *bytes_out);
 
 #ifdef __cplusplus
 }

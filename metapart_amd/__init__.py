@@ -56,9 +56,19 @@ class FinishStats(ct.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class SampleResult(ct.Structure):
+    _fields_ = [("reads", ct.c_uint64), ("tuples", ct.c_uint64), ("components", ct.c_uint64),
+                ("largest", ct.c_uint64), ("W", ct.c_uint64 * (NPART + 1))]
+
+    def as_dict(self):
+        return {"reads": self.reads, "tuples": self.tuples, "components": self.components, "largest": self.largest,
+                "W": np.array(self.W[:], dtype=np.uint64)}
+
+
 _lib = None
 _synth = None
 P64 = ct.POINTER(ct.c_uint64)
+MAX_SAMPLES = 256
 
 
 def lib():
@@ -88,6 +98,11 @@ def lib():
             "mp_get_bad": (i, [vp, vp]),
             "mp_dump_tuples": (i, [vp, vp, u64, P64]),
             "mp_write_partitions": (i, [vp, cp, i, P64, ct.POINTER(ct.c_int64), P64]),
+            "mp_batch_load": (i, [vp, i, ct.POINTER(vp), P64]),
+            "mp_batch_load_device": (i, [vp, vp, u64, i, P64]),
+            "mp_batch_partition": (i, [vp, ct.POINTER(SampleResult), ct.POINTER(ScanStats), ct.POINTER(FinishStats)]),
+            "mp_batch_get_parent": (i, [vp, i, vp]),
+            "mp_batch_write": (i, [vp, cp, P64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -101,7 +116,8 @@ def lib():
 SYMBOLS = ["mp_index_fastq", "mp_index_buffer", "mp_plan_ranges", "mp_ctx_create", "mp_ctx_destroy",
            "mp_last_error", "mp_ctx_device_bytes", "mp_count_chunks", "mp_count_loaded", "mp_chunk_first_ids",
            "mp_load_reads", "mp_load_reads_device", "mp_scan", "mp_finish", "mp_get_parent", "mp_get_bad", "mp_dump_tuples",
-           "mp_write_partitions"]
+           "mp_write_partitions", "mp_batch_load", "mp_batch_load_device", "mp_batch_partition", "mp_batch_get_parent",
+           "mp_batch_write"]
 
 
 def _u64(a) -> np.ndarray:
@@ -263,6 +279,44 @@ class Context:
         return int(nb.value)
 
 
+    # ------------------------------------------------------------ batches of samples
+    def batch_load(self, bufs):
+        """Upload n independent single-end FASTQ samples (host buffers) back to back."""
+        ptrs, sizes = zip(*[_buf_ptr(b) for b in bufs])
+        arr = (ct.c_void_p * len(bufs))(*ptrs)
+        sz = _u64(sizes)
+        self._check(lib().mp_batch_load(self._h, len(bufs), arr, _p(sz)), "mp_batch_load")
+        self.C = len(bufs)
+        self.N = int(self.first_ids()[-1])
+
+    def batch_load_device(self, dev_ptr: int, nbytes: int, sample_ofs):
+        """Samples already back to back in device memory; sample_ofs has n+1 byte offsets."""
+        so = _u64(sample_ofs)
+        self._check(lib().mp_batch_load_device(self._h, ct.c_void_p(dev_ptr), nbytes, len(so) - 1, _p(so)),
+                    "mp_batch_load_device")
+        self.C = len(so) - 1
+        self.N = int(self.first_ids()[-1])
+
+    def batch_partition(self):
+        """(per-sample result dicts, ScanStats, FinishStats) for the loaded batch."""
+        res = (SampleResult * self.C)()
+        st, fs = ScanStats(), FinishStats()
+        self._check(lib().mp_batch_partition(self._h, res, ct.byref(st), ct.byref(fs)), "mp_batch_partition")
+        return [r.as_dict() for r in res], st, fs
+
+    def batch_parent(self, s: int) -> np.ndarray:
+        ids = self.first_ids()
+        out = np.zeros(int(ids[s + 1] - ids[s]), dtype=np.uint32)
+        self._check(lib().mp_batch_get_parent(self._h, s, out.ctypes.data), "mp_batch_get_parent")
+        return out
+
+    def batch_write(self, out_dir: Optional[str]) -> int:
+        nb = ct.c_uint64(0)
+        self._check(lib().mp_batch_write(self._h, out_dir.encode() if out_dir is not None else None, ct.byref(nb)),
+                    "mp_batch_write")
+        return int(nb.value)
+
+
 @dataclass
 class RunResult:
     W: np.ndarray
@@ -348,6 +402,26 @@ def synth_cfg(preset: int = 0, **over) -> SynthCfg:
         else:
             setattr(c, k, v)
     return c
+
+
+def mix_params(n_samples: int = 64, seed: int = 4, reads_min: int = 10_000, reads_max: int = 5_000_000,
+               genomes_min: int = 20, genomes_max: int = 300, genome_len_min: int = 100_000,
+               genome_len_max: int = 1_000_000, len_min: int = 100, len_max: int = 150, err_min: float = 0.005,
+               err_max: float = 0.02):
+    """Per-sample generator settings of the 64-sample mix: every sample has its
+    own seeded genome pool (genomes_min..genomes_max genomes), a log-uniform
+    read count (even), one read length in [len_min, len_max] and a uniform
+    error rate in [err_min, err_max]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_samples):
+        n = int(np.exp(rng.uniform(np.log(reads_min), np.log(reads_max))))
+        n += n & 1
+        out.append(dict(seed=1000003 * (seed + 1) + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
+                        genome_len_min=genome_len_min, genome_len_max=genome_len_max, abund_sigma=1.0,
+                        n_reads=n, read_len=int(rng.integers(len_min, len_max + 1)),
+                        err_rate=float(rng.uniform(err_min, err_max))))
+    return out
 
 
 def synth(preset: int = 0, **over) -> SynthBuffer:

@@ -79,6 +79,8 @@ static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *c
     ctx->finished = false;
     ctx->scans_done = 0;
     ctx->last_n = 0;
+    ctx->batch = false;
+    ctx->sample_W.clear();
     MP_TRY(parse_reads(ctx, chunk_ofs, chunk_first_id, C));
     ctx->chunk_ofs.assign(chunk_ofs, chunk_ofs + C + 1);
     const uint64_t N = ctx->N;
@@ -276,6 +278,149 @@ int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n)
         memcpy(t + 10, &l[i], 4);
     }
     return 0;
+}
+
+// ---------------------------------------------------------------- batches of samples
+static int batch_after_load(mp_ctx *ctx)
+{
+    for (int c = 0; c < ctx->C; c++) {
+        const uint64_t nr = ctx->chunk_ids[c + 1] - ctx->chunk_ids[c];
+        if (nr == 0 || (nr & 1))
+            return ctx->fail(MP_EINVAL, "sample %d holds %llu records (need an even, non-zero count)", c,
+                             (unsigned long long)nr);
+    }
+    ctx->batch = true;
+    return 0;
+}
+
+int mp_batch_load(mp_ctx *ctx, int n, const char *const *fq, const uint64_t *nbytes)
+{
+    if (!ctx || n <= 0 || !fq || !nbytes) return MP_EINVAL;
+    if (n > MP_MAX_SAMPLES) return ctx->fail(MP_EINVAL, "at most %d samples per batch", MP_MAX_SAMPLES);
+    MP_HIP(hipSetDevice(ctx->device));
+    std::vector<uint64_t> ofs(n + 1, 0);
+    for (int i = 0; i < n; i++) ofs[i + 1] = ofs[i] + nbytes[i];
+    MP_TRY(ensure(ctx, ctx->fq, ofs[n] + 64));
+    for (int i = 0; i < n; i++)
+        if (nbytes[i])
+            MP_HIP(hipMemcpyAsync((uint8_t *)ctx->fq.p + ofs[i], fq[i], nbytes[i], hipMemcpyHostToDevice, ctx->stream));
+    MP_TRY(fill_u32(ctx, (uint8_t *)ctx->fq.p + ((ofs[n] + 3) & ~3ull), 0, 8));
+    ctx->d_fq = (const uint8_t *)ctx->fq.p;
+    ctx->fq_owned = true;
+    ctx->fq_bytes = ofs[n];
+    MP_TRY(load_common(ctx, ofs.data(), nullptr, n));
+    return batch_after_load(ctx);
+}
+
+int mp_batch_load_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, int n, const uint64_t *sample_ofs)
+{
+    if (!ctx || !d_fq || n <= 0 || !sample_ofs) return MP_EINVAL;
+    if (n > MP_MAX_SAMPLES) return ctx->fail(MP_EINVAL, "at most %d samples per batch", MP_MAX_SAMPLES);
+    MP_HIP(hipSetDevice(ctx->device));
+    ctx->d_fq = (const uint8_t *)d_fq;
+    ctx->fq_owned = false;
+    ctx->fq_bytes = nbytes;
+    MP_TRY(load_common(ctx, sample_ofs, nullptr, n));
+    return batch_after_load(ctx);
+}
+
+int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_stats, mp_finish_stats *finish_stats)
+{
+    if (!ctx) return MP_EINVAL;
+    if (!ctx->batch) return ctx->fail(MP_ESTATE, "mp_batch_partition before mp_batch_load");
+    MP_HIP(hipSetDevice(ctx->device));
+    const uint64_t N = ctx->N;
+    const int ns = ctx->C;
+    mp_scan_stats st;
+    memset(&st, 0, sizeof(st));
+    MP_TRY(iota_u32(ctx, (uint32_t *)ctx->parent.p, N));
+    MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
+    MP_TRY(fill_u32(ctx, ctx->filtered.p, 0, (N + 3) / 4));
+    ctx->scans_done = 0;
+    hipEventRecord(ctx->ev[0], ctx->stream);
+    uint64_t n = 0;
+    MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n));
+    hipEventRecord(ctx->ev[1], ctx->stream);
+    // sort by (sample, k-mer, adj): the sample of a label is its chunk
+    uint64_t *d_first = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_first, sizeof(uint64_t) * (ns + 1), ctx->stream));
+    MP_HIP(hipMemcpyAsync(d_first, ctx->chunk_ids.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    std::vector<uint64_t> tpos(ns + 1, 0);
+    int passes = 0;
+    const int rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x3FF, &passes, d_first, ns,
+                                      tpos.data());
+    MP_HIP(hipFreeAsync(d_first, ctx->stream));
+    MP_TRY(rc);
+    if (n <= 1) {  // the sort returned early: one tuple or none, all in the sample that owns it
+        uint32_t l0 = 0;
+        if (n == 1) {
+            MP_HIP(hipMemcpyAsync(&l0, ctx->lab[ctx->cur].p, 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        for (int i = 0; i <= ns; i++) tpos[i] = (n == 1 && ctx->chunk_ids[i] > l0) ? 1 : 0;
+        tpos[ns] = n;
+    }
+    for (int i = 0; i < ns; i++)  // a plain run's range plan is degenerate without k-mers (mp_plan_ranges)
+        if (tpos[i + 1] == tpos[i]) return ctx->fail(MP_EPLAN, "sample %d has no k-mers to partition", i);
+    hipEventRecord(ctx->ev[7], ctx->stream);
+    // one filter/union segment per sample, each its own quirk group
+    std::vector<uint64_t> grp(2 * (size_t)ns);
+    for (int i = 0; i < ns; i++) {
+        grp[2 * i] = tpos[i];
+        grp[2 * i + 1] = tpos[i + 1];
+    }
+    MP_TRY(filter_union_groups(ctx, n, tpos.data(), ns, grp.data(), &st));
+    st.tuples = n;
+    st.sort_passes = passes;
+    st.ms_sort_pass_avg = ctx->sort_pass_avg;
+    st.ms_sort_hist = ctx->sort_hist_ms;
+    hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
+    hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
+    ctx->last_n = n;
+    ctx->have_scan = true;
+    ctx->scans_done = 1;
+    ctx->sample_tuples = tpos;
+    mp_finish_stats fs;
+    memset(&fs, 0, sizeof(fs));
+    uint64_t W[MP_NPART + 1];
+    MP_TRY(finish_components(ctx, W, &fs));
+    ctx->finished = true;
+    if (res) {
+        for (int i = 0; i < ns; i++) {
+            mp_sample_result &r = res[i];
+            r.reads = ctx->chunk_ids[i + 1] - ctx->chunk_ids[i];
+            r.tuples = tpos[i + 1] - tpos[i];
+            r.components = ctx->sample_comp[i];
+            r.largest = ctx->sample_largest[i];
+            for (int p = 0; p <= MP_NPART; p++) r.W[p] = ctx->sample_W[(size_t)i * (MP_NPART + 1) + p];
+        }
+    }
+    if (scan_stats) *scan_stats = st;
+    if (finish_stats) *finish_stats = fs;
+    return 0;
+}
+
+int mp_batch_get_parent(mp_ctx *ctx, int s, uint32_t *parent)
+{
+    if (!ctx || !parent) return MP_EINVAL;
+    if (!ctx->batch || !ctx->finished) return ctx->fail(MP_ESTATE, "no partitioned batch");
+    if (s < 0 || s >= ctx->C) return ctx->fail(MP_EINVAL, "sample %d out of range", s);
+    MP_HIP(hipSetDevice(ctx->device));
+    const uint64_t a = ctx->chunk_ids[s], b = ctx->chunk_ids[s + 1];
+    MP_HIP(hipMemcpyAsync(parent, (const uint32_t *)ctx->parent.p + a, (b - a) * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t i = 0; i < b - a; i++) parent[i] -= (uint32_t)a;
+    return 0;
+}
+
+int mp_batch_write(mp_ctx *ctx, const char *out_dir, uint64_t *bytes_out)
+{
+    if (!ctx) return MP_EINVAL;
+    if (!ctx->batch || !ctx->finished) return ctx->fail(MP_ESTATE, "no partitioned batch");
+    MP_HIP(hipSetDevice(ctx->device));
+    return write_partitions(ctx, out_dir, 1, nullptr, nullptr, bytes_out);
 }
 
 int mp_write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,

@@ -250,10 +250,6 @@ __global__ void first_alive_kernel(GParams G, uint64_t *__restrict__ first)
 }
 
 __global__ void quirk_union_kernel(uint32_t *parent, uint32_t a, uint32_t b) { uf_union(parent, a, b); }
-__global__ void quirk_fix_kernel(uint32_t *parent, uint8_t *bad, uint32_t L0)
-{
-    if (bad[L0] && parent[L0] != L0) bad[L0] = 0;
-}
 
 __global__ void compress_kernel(uint32_t *parent, const uint8_t *__restrict__ bad, uint64_t N)
 {
@@ -396,43 +392,118 @@ __global__ void pair_merge_kernel(uint32_t *__restrict__ parent, const uint32_t 
     if (i < N && gsize[gparent[i]] > MP_PAIR_GATE) parent[i] = host_parent[i];
 }
 
+// Per-sample partition plans of a batch: W[p] over the sample's own read ids.
+__global__ void sample_plan_kernel(const uint64_t *__restrict__ excl, const uint64_t *__restrict__ first,
+                                   uint64_t *__restrict__ W)
+{
+    const int s = blockIdx.x, p = threadIdx.x;
+    if (p > MP_NPART) return;
+    const uint64_t off = first[s], n = first[s + 1] - off, base = excl[off];
+    uint64_t *w = W + (size_t)s * (MP_NPART + 1);
+    if (p == 0) { w[0] = 0; return; }
+    if (p == MP_NPART) { w[p] = n; return; }
+    const uint64_t target = (uint64_t)p * (n / MP_NPART);
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (excl[off + mid] - base < target) lo = mid + 1; else hi = mid;
+    }
+    w[p] = lo;
+}
+
+// Per-sample root counts and largest components of a batch.
+__global__ __launch_bounds__(256) void sample_stats_kernel(const uint32_t *__restrict__ parent,
+                                                           const uint32_t *__restrict__ sizes,
+                                                           const uint64_t *__restrict__ first, int ns, uint64_t N,
+                                                           unsigned long long *__restrict__ out)
+{
+    __shared__ uint32_t s_n[4], s_m[4];
+    const uint64_t b0 = (uint64_t)blockIdx.x * blockDim.x;
+    const uint64_t i = b0 + threadIdx.x;
+    const uint64_t bl = min(b0 + blockDim.x, N) - 1;
+    int s = 0, hi = ns - 1;
+    const uint64_t x = i < N ? i : bl;
+    while (s < hi) {
+        const int mid = (s + hi + 1) >> 1;
+        if (first[mid] <= x) s = mid; else hi = mid - 1;
+    }
+    const bool root = i < N && parent[i] == (uint32_t)i;
+    const uint32_t sz = root ? sizes[i] : 0;
+    // one sample in the whole workgroup (the common case): reduce, one atomic each
+    const bool uniform = __syncthreads_and(first[s] <= b0 && bl < first[s + 1]);
+    if (uniform) {
+        const uint64_t m = __ballot(root);
+        uint32_t mx = sz;
+        for (int d = kWave / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_down(mx, d, kWave));
+        if (__lane_id() == 0) {
+            s_n[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+            s_m[threadIdx.x / kWave] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t a = 0, b = 0;
+            for (int w = 0; w < 4; w++) {
+                a += s_n[w];
+                b = max(b, s_m[w]);
+            }
+            if (a) atomicAdd(&out[2 * s], (unsigned long long)a);
+            if (b) atomicMax(&out[2 * s + 1], (unsigned long long)b);
+        }
+    } else if (root) {
+        atomicAdd(&out[2 * s], 1ull);
+        atomicMax(&out[2 * s + 1], (unsigned long long)sz);
+    }
+}
+
 static inline unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st)
+// Per segment: first surviving tuple, its k-mer and label, and the last k-mer,
+// first label and that label's BAD flag of the segment's quirk group.
+__global__ void quirk_info_kernel(GParams G, const uint64_t *__restrict__ first, const uint64_t *__restrict__ grp,
+                                  uint64_t *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G.T) return;
+    const uint64_t f = first[t], gs = grp[2 * t], ge = grp[2 * t + 1];
+    uint64_t *o = out + 5 * (uint64_t)t;
+    o[0] = f;
+    o[1] = f != ~0ull ? G.keys[f] : 0;
+    o[2] = f != ~0ull ? G.lab[f] : 0;
+    o[3] = ge > gs ? G.keys[ge - 1] : 0;
+    const uint32_t L0 = ge > gs ? G.lab[gs] : 0;
+    o[4] = (uint64_t)L0 | ((uint64_t)(ge > gs && G.bad[L0]) << 32) | ((uint64_t)(ge > gs) << 33);
+}
+
+// Filter, union and compress over the n sorted tuples of one scan, split into
+// nseg segments at device positions d_seg[0..nseg].  grp (host, 2 per segment)
+// is the tuple range whose last k-mer and first label drive the segment's
+// first-run quirk: the whole scan for one sample, the sample's own tuples in
+// a batch of samples.
+static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int nseg, const uint64_t *grp,
+                             mp_scan_stats *st)
 {
     const uint64_t N = ctx->N;
-    const int bshift = 2 * (ctx->cfg.k - 10);
     uint64_t *keys = (uint64_t *)ctx->keys[ctx->cur].p;
     const uint32_t *lab = (const uint32_t *)ctx->lab[ctx->cur].p;
     const uint8_t *adj = (const uint8_t *)ctx->adj[ctx->cur].p;
     const uint64_t nwords = (n + 31) / 32 + 1;
     MP_TRY(ensure(ctx, ctx->zbits, nwords * sizeof(uint32_t)));
     MP_TRY(fill_u32(ctx, ctx->zbits.p, 0, nwords));
-    // small device scratch: bounds, positions, stats, first-alive
+    // small device scratch: first-alive, quirk groups and info, stats
     uint64_t *d_small = nullptr;
-    const size_t small_words = 3 * (size_t)(T + 1) + 16;
+    const size_t small_words = 8 * (size_t)nseg + 16;
     MP_HIP(hipMallocAsync((void **)&d_small, small_words * sizeof(uint64_t), ctx->stream));
-    uint64_t *d_bounds = d_small, *d_seg = d_small + (T + 1), *d_first = d_small + 2 * (T + 1);
-    unsigned long long *d_stats = (unsigned long long *)(d_small + 3 * (T + 1));
+    uint64_t *d_first = d_small, *d_grp = d_small + nseg, *d_info = d_small + 3 * nseg;
+    unsigned long long *d_stats = (unsigned long long *)(d_small + 8 * nseg);
     MP_TRY(fill_u32(ctx, d_stats, 0, 32));
-    MP_HIP(hipMemcpyAsync(d_bounds, seg_bounds, sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(seg_pos_kernel, dim3(nblk(T + 1)), dim3(256), 0, ctx->stream, (const uint64_t *)keys, n, bshift,
-                       (const uint64_t *)d_bounds, T, d_seg);
-    MP_HIP(hipGetLastError());
-    std::vector<uint64_t> segpos(T + 1);
-    MP_HIP(hipMemcpyAsync(segpos.data(), d_seg, sizeof(uint64_t) * (T + 1), hipMemcpyDeviceToHost, ctx->stream));
-    MP_HIP(hipStreamSynchronize(ctx->stream));
-    if (segpos[0] != 0 || segpos[T] != n) {
-        MP_HIP(hipFreeAsync(d_small, ctx->stream));
-        return ctx->fail(MP_EPLAN, "tuples outside the scan's thread ranges");
-    }
+    MP_HIP(hipMemcpyAsync(d_grp, grp, sizeof(uint64_t) * 2 * nseg, hipMemcpyHostToDevice, ctx->stream));
     GParams G;
     G.keys = keys;
     G.adj = adj;
     G.lab = lab;
     G.n = n;
     G.seg = d_seg;
-    G.T = T;
+    G.T = nseg;
     G.bad = (uint8_t *)ctx->bad.p;
     G.filtered = (uint8_t *)ctx->filtered.p;
     G.zbits = (uint32_t *)ctx->zbits.p;
@@ -451,34 +522,28 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
                        (const uint8_t *)G.filtered, G.parent, N, d_stats + 2);
     MP_HIP(hipGetLastError());
     hipEventRecord(ctx->ev[3], ctx->stream);
-    // first-run quirk (metapart.c:2539-2616): each segment starts with the
-    // global last k-mer as the "current" k-mer and tuple 0's label as centre
+    // first-run quirk (metapart.c:2539-2616): each segment starts with its
+    // group's last k-mer as the "current" k-mer and the group's first label as
+    // the centre, so a segment whose first surviving run has that k-mer joins
+    // its labels to that centre
     int quirks = 0;
     if (n) {
-        hipLaunchKernelGGL(first_alive_kernel, dim3(nblk(T)), dim3(256), 0, ctx->stream, G, d_first);
+        hipLaunchKernelGGL(first_alive_kernel, dim3(nblk(nseg)), dim3(256), 0, ctx->stream, G, d_first);
+        hipLaunchKernelGGL(quirk_info_kernel, dim3(nblk(nseg)), dim3(256), 0, ctx->stream, G,
+                           (const uint64_t *)d_first, (const uint64_t *)d_grp, d_info);
         MP_HIP(hipGetLastError());
-        std::vector<uint64_t> first(T);
-        uint64_t lastK = 0;
-        uint32_t L0 = 0;
-        MP_HIP(hipMemcpyAsync(first.data(), d_first, sizeof(uint64_t) * T, hipMemcpyDeviceToHost, ctx->stream));
-        MP_HIP(hipMemcpyAsync(&lastK, keys + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-        MP_HIP(hipMemcpyAsync(&L0, lab, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<uint64_t> info(5 * (size_t)nseg);
+        MP_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(uint64_t) * 5 * nseg, hipMemcpyDeviceToHost, ctx->stream));
         MP_HIP(hipStreamSynchronize(ctx->stream));
-        uint8_t L0bad = 0;
-        for (int t = 0; t < T; t++) {
-            if (first[t] == ~0ull) continue;
-            uint64_t kf = 0;
-            uint32_t lf = 0;
-            MP_HIP(hipMemcpyAsync(&kf, keys + first[t], sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(&lf, lab + first[t], sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(&L0bad, G.bad + L0, 1, hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipStreamSynchronize(ctx->stream));
-            if (kf != lastK) continue;
-            if (L0bad) {
+        for (int t = 0; t < nseg; t++) {
+            const uint64_t *o = &info[5 * (size_t)t];
+            if (o[0] == ~0ull || !((o[4] >> 33) & 1) || o[1] != o[3]) continue;
+            if ((o[4] >> 32) & 1) {
                 MP_HIP(hipFreeAsync(d_small, ctx->stream));
                 return ctx->fail(MP_EPLAN, "first-run union quirk with a filtered tuple-0 label (segment %d)", t);
             }
             quirks++;
+            const uint32_t L0 = (uint32_t)o[4], lf = (uint32_t)o[2];
             if (lf != L0) {
                 hipLaunchKernelGGL(quirk_union_kernel, dim3(1), dim3(1), 0, ctx->stream, G.parent, L0, lf);
                 MP_HIP(hipGetLastError());
@@ -512,6 +577,44 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
         st->ms_union = b;
     }
     return 0;
+}
+
+int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st)
+{
+    const int bshift = 2 * (ctx->cfg.k - 10);
+    uint64_t *d_small = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_small, 2 * (size_t)(T + 1) * sizeof(uint64_t), ctx->stream));
+    uint64_t *d_bounds = d_small, *d_seg = d_small + (T + 1);
+    MP_HIP(hipMemcpyAsync(d_bounds, seg_bounds, sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(seg_pos_kernel, dim3(nblk(T + 1)), dim3(256), 0, ctx->stream,
+                       (const uint64_t *)ctx->keys[ctx->cur].p, n, bshift, (const uint64_t *)d_bounds, T, d_seg);
+    MP_HIP(hipGetLastError());
+    std::vector<uint64_t> segpos(T + 1);
+    MP_HIP(hipMemcpyAsync(segpos.data(), d_seg, sizeof(uint64_t) * (T + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (segpos[0] != 0 || segpos[T] != n) {
+        MP_HIP(hipFreeAsync(d_small, ctx->stream));
+        return ctx->fail(MP_EPLAN, "tuples outside the scan's thread ranges");
+    }
+    std::vector<uint64_t> grp(2 * (size_t)T);
+    for (int t = 0; t < T; t++) {
+        grp[2 * t] = 0;
+        grp[2 * t + 1] = n;
+    }
+    const int rc = filter_union_core(ctx, n, d_seg, T, grp.data(), st);
+    MP_HIP(hipFreeAsync(d_small, ctx->stream));
+    return rc;
+}
+
+int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int nseg, const uint64_t *grp,
+                        mp_scan_stats *st)
+{
+    uint64_t *d_seg = nullptr;
+    MP_HIP(hipMallocAsync((void **)&d_seg, (size_t)(nseg + 1) * sizeof(uint64_t), ctx->stream));
+    MP_HIP(hipMemcpyAsync(d_seg, seg_pos, sizeof(uint64_t) * (nseg + 1), hipMemcpyHostToDevice, ctx->stream));
+    const int rc = filter_union_core(ctx, n, d_seg, nseg, grp, st);
+    MP_HIP(hipFreeAsync(d_seg, ctx->stream));
+    return rc;
 }
 
 // ---------------------------------------------------------------- finish
@@ -633,6 +736,34 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     MP_HIP(hipGetLastError());
     uint64_t res[64];
     MP_HIP(hipMemcpyAsync(res, d_small, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
+    if (ctx->batch) {  // plans and statistics per sample
+        const int ns = ctx->C;
+        uint64_t *d_b = nullptr;
+        const size_t words = (size_t)(ns + 1) + (size_t)ns * (MP_NPART + 1) + 2 * (size_t)ns;
+        MP_HIP(hipMallocAsync((void **)&d_b, words * sizeof(uint64_t), ctx->stream));
+        uint64_t *d_first = d_b, *d_W = d_b + (ns + 1), *d_cs = d_W + (size_t)ns * (MP_NPART + 1);
+        MP_HIP(hipMemcpyAsync(d_first, ctx->chunk_ids.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+        MP_TRY(fill_u32(ctx, d_cs, 0, 4 * (uint64_t)ns));
+        hipLaunchKernelGGL(sample_plan_kernel, dim3(ns), dim3(64), 0, ctx->stream, (const uint64_t *)ctx->wg_off.p,
+                           (const uint64_t *)d_first, d_W);
+        hipLaunchKernelGGL(sample_stats_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+                           (const uint32_t *)sizes, (const uint64_t *)d_first, ns, N, (unsigned long long *)d_cs);
+        MP_HIP(hipGetLastError());
+        ctx->sample_W.resize((size_t)ns * (MP_NPART + 1));
+        std::vector<uint64_t> cs(2 * (size_t)ns);
+        MP_HIP(hipMemcpyAsync(ctx->sample_W.data(), d_W, sizeof(uint64_t) * ctx->sample_W.size(),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipMemcpyAsync(cs.data(), d_cs, sizeof(uint64_t) * cs.size(), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipFreeAsync(d_b, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->sample_comp.resize(ns);
+        ctx->sample_largest.resize(ns);
+        for (int i = 0; i < ns; i++) {
+            ctx->sample_comp[i] = cs[2 * i];
+            ctx->sample_largest[i] = cs[2 * i + 1];
+        }
+    }
     MP_HIP(hipFreeAsync(d_small, ctx->stream));
     hipEventRecord(ctx->ev[6], ctx->stream);
     MP_HIP(hipStreamSynchronize(ctx->stream));

@@ -111,6 +111,12 @@ struct mp_ctx {
     bool finished = false;
     int scans_done = 0;
     std::vector<uint64_t> chunk_ids;  // first read id of each chunk (C+1)
+    // batch of independent samples: one chunk per sample
+    bool batch = false;
+    std::vector<uint64_t> sample_W;        // 17 per sample, local read ids
+    std::vector<uint64_t> sample_comp;     // components per sample
+    std::vector<uint64_t> sample_largest;  // largest component per sample
+    std::vector<uint64_t> sample_tuples;   // tuple offsets of the samples (C+1)
 
     int fail(int code, const char *fmt, ...)
     {
@@ -155,11 +161,18 @@ int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot)
 int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out);
 // Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:
-// 0 = adj byte, 1..8 = key byte p-1.  *cur selects the buffer holding the data.
+// 0 = adj byte, 1..8 = key byte p-1, 9 = sample of the label (d_grp_first:
+// n_grp + 1 first read ids on the device; grp_base, if given, receives the
+// n_grp + 1 tuple offsets of the samples).  *cur selects the buffer holding the data.
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
-                       uint32_t pass_mask, int *passes_out);
+                       uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
+                       uint64_t *grp_base);
 int iota_u32(mp_ctx *ctx, uint32_t *p, uint64_t n);
 int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st);
+// Same over segments at the given tuple positions (nseg + 1, host) with
+// per-segment quirk groups (2 per segment: first and end tuple, host).
+int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int nseg, const uint64_t *grp,
+                        mp_scan_stats *st);
 int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st);
 int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
                      uint64_t *bytes_out);

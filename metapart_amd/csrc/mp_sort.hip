@@ -10,9 +10,9 @@
 // MI355X design: structure-of-arrays tuples (keys u64, labels u32, adj u8;
 // 13 bytes), one histogram read for every pass up front, then one "onesweep"
 // kernel per 8-bit digit:
-//   - 512 lanes take a 4096-tuple tile (tile ids from an atomic counter, so
-//     earlier tiles are always resident or done),
-//   - each wave ranks its 512 tuples stably with 8 ballots per item
+//   - 1024 lanes take a 6144-tuple tile (tile ids from an atomic counter, so
+//     earlier tiles are always resident or done); two tiles fit a CU,
+//   - each wave ranks its 384 tuples stably with 8 ballots per item
 //     (bit-sliced digit match) and per-wave digit counters in LDS,
 //   - per-digit tile counts are published and the digit's global prefix is
 //     found by decoupled look-back over earlier tiles (epoch-tagged status
@@ -21,6 +21,11 @@
 //     lanes write consecutive addresses of each digit's run.
 // Passes whose digit is constant over the scan (e.g. the top byte when K<29)
 // are skipped: a stable pass on a constant digit is the identity.
+//
+// Batches of samples add a last pass whose digit is the sample of the tuple's
+// label (labels are read ids, samples are contiguous id ranges): the result is
+// every sample's tuples in (k-mer, adj) order, samples one after another, the
+// same as sorting each sample alone.
 #include <cstdlib>
 
 #include "mp_internal.h"
@@ -28,7 +33,7 @@
 namespace mp {
 
 constexpr int kRadix = 256;
-constexpr int kPasses = 9;
+constexpr int kPasses = 10;  // adj, k-mer bytes 0..7, sample of the label
 
 constexpr uint64_t kStAgg = 1, kStInc = 2;
 constexpr uint64_t kStVal = (1ull << 46) - 1;
@@ -46,16 +51,31 @@ struct SortPass {
     uint32_t *lab_out;
     uint8_t *adj_out;
     uint64_t n;
-    int shift;  // < 0: the digit is the adj byte
+    int shift;  // -1: the digit is the adj byte, -2: the sample of the label
+    const uint64_t *grp_first;  // n_grp + 1 first read ids of the samples
+    int n_grp;
     const uint64_t *digit_base;
     uint64_t *status;
     uint32_t *tile_ctr;
     uint32_t epoch;
 };
 
+// sample of a read id: last g with first[g] <= x
+__device__ __forceinline__ uint32_t group_of(const uint64_t *first, int n, uint32_t x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return (uint32_t)lo;
+}
+
 __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__restrict__ keys,
-                                                           const uint8_t *__restrict__ adj, uint64_t n,
-                                                           unsigned long long *__restrict__ ghist)
+                                                           const uint8_t *__restrict__ adj,
+                                                           const uint32_t *__restrict__ lab,
+                                                           const uint64_t *__restrict__ grp_first, int n_grp,
+                                                           uint64_t n, unsigned long long *__restrict__ ghist)
 {
     __shared__ uint32_t h[kPasses][kRadix];
     for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) (&h[0][0])[j] = 0;
@@ -66,6 +86,7 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
         atomicAdd(&h[0][adj[i]], 1u);
 #pragma unroll
         for (int b = 0; b < 8; b++) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
+        if (grp_first) atomicAdd(&h[9][group_of(grp_first, n_grp, lab[i])], 1u);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) {
@@ -117,7 +138,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
             k[i] = P.keys_in[idx];
             l[i] = P.lab_in[idx];
             a[i] = P.adj_in[idx];
-            d[i] = digit_of(k[i], a[i], P.shift);
+            d[i] = P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a[i], P.shift);
         } else {
             k[i] = 0;
             l[i] = 0;
@@ -235,7 +256,8 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
 }
 
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
-                       uint32_t pass_mask, int *passes_out)
+                       uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
+                       uint64_t *grp_base)
 {
     *passes_out = 0;
     if (n <= 1) return 0;
@@ -249,13 +271,15 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         uint64_t blocks = (n + kRadix * 16 - 1) / (kRadix * 16);
         if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)blocks), dim3(kRadix), 0, ctx->stream,
-                           (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p, n, gh);
+                           (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p,
+                           (const uint32_t *)lab[*cur].p, d_grp_first, n_grp, n, gh);
         MP_HIP(hipGetLastError());
     }
     hipEventRecord(ctx->pev[1], ctx->stream);
     uint64_t h[kPasses * kRadix];
     MP_HIP(hipMemcpyAsync(h, gh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (!d_grp_first) h[9 * kRadix] = n;  // no samples: one constant digit
     // digit base offsets of the passes that do work
     uint64_t base[kPasses][kRadix];
     bool run[kPasses];
@@ -270,6 +294,10 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
             return ctx->fail(MP_EDEVICE, "sort histogram total %llu != %llu", (unsigned long long)acc,
                              (unsigned long long)n);
         run[p] = ((pass_mask >> p) & 1) && mx != n;
+    }
+    if (n_grp > 0 && grp_base) {  // tuple offset of every sample after the sort
+        for (int g = 0; g < n_grp; g++) grp_base[g] = base[9][g];
+        grp_base[n_grp] = n;
     }
     // tile shape: lanes x items.  Default from a sweep on MI355X at 2.4e9
     // tuples: 1024 x 6 (two tiles per CU, 22.8 ms/pass) < 1024 x 8 (one
@@ -310,7 +338,9 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.lab_out = (uint32_t *)lab[dst].p;
         P.adj_out = (uint8_t *)adj[dst].p;
         P.n = n;
-        P.shift = p == 0 ? -1 : 8 * (p - 1);
+        P.shift = p == 0 ? -1 : p == 9 ? -2 : 8 * (p - 1);
+        P.grp_first = d_grp_first;
+        P.n_grp = n_grp;
         P.digit_base = d_base + p * kRadix;
         P.status = (uint64_t *)ctx->status.p;
         P.tile_ctr = tile_ctr + executed;
@@ -344,7 +374,8 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
 
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
 {
-    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FF, passes_out);
+    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FF, passes_out, nullptr, 0,
+                              nullptr);
 }
 
 }  // namespace mp

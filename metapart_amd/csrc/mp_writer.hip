@@ -8,7 +8,11 @@
 // tuple sort (so each file keeps chunk and record order), byte offsets come
 // from a scan over item lengths in that order, and one wave per item copies
 // the record bytes into a single output buffer holding the 16*T files back to
-// back.  The host then writes each file's slice.
+// back.  The host then writes each file's slice.  A batch of samples (one
+// chunk each) gets 16 files per sample, planned over each sample's own ids.
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -25,6 +29,7 @@ struct WParams {
     const uint64_t *chunk_ids;  // C+1
     const uint64_t *pair_from;  // per chunk: first record that starts pairs (or chunk end)
     const uint64_t *W;          // 17
+    const uint64_t *Ws;         // batch of samples (one chunk each): 17 per sample, else null
     uint64_t N;
     int C, T, cpt;
     uint64_t *file;   // 2N item keys
@@ -75,16 +80,20 @@ __global__ void items_kernel(WParams P)
     const int c = lo;
     const int t = c / P.cpt;
     const uint64_t pf = P.pair_from[c];
+    // a sample of a batch has its own plan over its own read ids and files
+    const uint64_t *W = P.Ws ? P.Ws + (size_t)c * (MP_NPART + 1) : P.W;
+    const uint64_t id0 = P.Ws ? P.chunk_ids[c] : 0;
+    const uint32_t fbase = P.Ws ? (uint32_t)c * MP_NPART * P.T : 0;
     uint32_t f0 = kNoFile, f1 = kNoFile, l0 = 0;
     const uint64_t pu = P.parent[r];
     if (r < pf) {
-        f0 = (uint32_t)(part_of(P.W, pu) * P.T + t);
+        f0 = fbase + (uint32_t)(part_of(W, pu - id0) * P.T + t);
         l0 = (uint32_t)(P.rec_off[r + 1] - P.rec_off[r]);
     } else if (((r - pf) & 1) == 0) {
         const uint64_t mate = (r & 1) == 0 ? r + 1 : r - 1;  // metapart.c:1013-1016
         const uint64_t pv = P.parent[mate];
-        f0 = (uint32_t)(part_of(P.W, pu) * P.T + t);
-        if (pu != pv) f1 = (uint32_t)(part_of(P.W, pv) * P.T + t);
+        f0 = fbase + (uint32_t)(part_of(W, pu - id0) * P.T + t);
+        if (pu != pv) f1 = fbase + (uint32_t)(part_of(W, pv - id0) * P.T + t);
         l0 = (uint32_t)(P.rec_off[r + 2] - P.rec_off[r]);
     }
     P.file[2 * r] = f0;
@@ -119,7 +128,9 @@ __global__ void copy_items_kernel(const uint8_t *__restrict__ fq, const uint64_t
     for (uint64_t b = lane; b < len; b += kWave) out[dst + b] = fq[src + b];
 }
 
-__global__ void file_bounds_kernel(const uint64_t *__restrict__ file, uint64_t n, int F, uint64_t *__restrict__ pos)
+// first item of every file (and the end), and the byte offset there
+__global__ void file_bounds_kernel(const uint64_t *__restrict__ file, uint64_t n, int F, const uint64_t *__restrict__ off,
+                                   uint64_t *__restrict__ pos, uint64_t *__restrict__ byte)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f > F) return;
@@ -129,6 +140,19 @@ __global__ void file_bounds_kernel(const uint64_t *__restrict__ file, uint64_t n
         if (file[mid] < (uint64_t)f) lo = mid + 1; else hi = mid;
     }
     pos[f] = lo;
+    byte[f] = off[lo];
+}
+
+static int make_dirs(const std::string &path)
+{
+    std::string cur;
+    for (size_t i = 0; i <= path.size(); i++) {
+        if (i == path.size() || path[i] == '/') {
+            if (!cur.empty() && mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST) return -1;
+        }
+        if (i < path.size()) cur.push_back(path[i]);
+    }
+    return 0;
 }
 
 int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
@@ -136,22 +160,28 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
 {
     const uint64_t N = ctx->N;
     const int C = ctx->C;
-    const int F = MP_NPART * T;
+    const bool batch = ctx->batch;
+    const int F = MP_NPART * T * (batch ? C : 1);
     if (F >= (int)kNoFile) return ctx->fail(MP_EINVAL, "too many output files");
+    if (batch && (T != 1 || ctx->sample_W.size() != (size_t)C * (MP_NPART + 1)))
+        return ctx->fail(MP_ESTATE, "batch output needs T = 1 and per-sample plans");
     std::vector<int64_t> il(C, -1);
     if (il_start)
         for (int c = 0; c < C; c++) il[c] = il_start[c];
     // small device inputs
-    const size_t nsmall = (size_t)(C + 1) * 2 + C + 17 + (size_t)C + (F + 1);
+    const size_t nW = batch ? (size_t)C * (MP_NPART + 1) : (size_t)(MP_NPART + 1);
+    const size_t nsmall = (size_t)(C + 1) * 2 + C + nW + (size_t)C + 2 * (size_t)(F + 1);
     uint64_t *d_small = nullptr;
     MP_HIP(hipMallocAsync((void **)&d_small, nsmall * sizeof(uint64_t), ctx->stream));
     uint64_t *d_ids = d_small, *d_cofs = d_small + (C + 1), *d_pf = d_small + 2 * (C + 1);
     uint64_t *d_W = d_pf + C;
-    int64_t *d_il = (int64_t *)(d_W + 17);
+    int64_t *d_il = (int64_t *)(d_W + nW);
     uint64_t *d_fpos = (uint64_t *)(d_il + C);
+    uint64_t *d_fbyte = d_fpos + (F + 1);
     MP_HIP(hipMemcpyAsync(d_ids, ctx->chunk_ids.data(), sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice, ctx->stream));
     MP_HIP(hipMemcpyAsync(d_cofs, ctx->chunk_ofs.data(), sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice, ctx->stream));
-    MP_HIP(hipMemcpyAsync(d_W, W, sizeof(uint64_t) * 17, hipMemcpyHostToDevice, ctx->stream));
+    MP_HIP(hipMemcpyAsync(d_W, batch ? ctx->sample_W.data() : W, sizeof(uint64_t) * nW, hipMemcpyHostToDevice,
+                          ctx->stream));
     MP_HIP(hipMemcpyAsync(d_il, il.data(), sizeof(int64_t) * C, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(pair_from_kernel, dim3((C + 255) / 256), dim3(256), 0, ctx->stream,
                        (const uint64_t *)ctx->rec_off.p, (const uint64_t *)d_ids, (const uint64_t *)d_cofs,
@@ -180,6 +210,7 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     P.chunk_ids = d_ids;
     P.pair_from = d_pf;
     P.W = d_W;
+    P.Ws = batch ? d_W : nullptr;
     P.N = N;
     P.C = C;
     P.T = T;
@@ -192,7 +223,8 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     hipLaunchKernelGGL(items_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, P);
     MP_HIP(hipGetLastError());
     int cur = 0, passes = 0;
-    MP_TRY(radix_sort_generic(ctx, ctx->items, ctx->ilens, ctx->iaux, &cur, n_items, 0x6 /* key bytes 0,1 */, &passes));
+    MP_TRY(radix_sort_generic(ctx, ctx->items, ctx->ilens, ctx->iaux, &cur, n_items, 0x6 /* key bytes 0,1 */, &passes,
+                                  nullptr, 0, nullptr));
     const uint64_t *file = (const uint64_t *)ctx->items[cur].p;
     const uint32_t *slot = (const uint32_t *)ctx->ilens[cur].p;
     // lengths in sorted order (reuse the other ilens buffer), offsets by scan
@@ -202,20 +234,15 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     MP_HIP(hipGetLastError());
     uint64_t *off = (uint64_t *)ctx->wg_off.p;
     MP_TRY(scan_u32(ctx, slen, off, n_items));
-    hipLaunchKernelGGL(file_bounds_kernel, dim3((F + 256) / 256), dim3(256), 0, ctx->stream, file, n_items, F, d_fpos);
+    hipLaunchKernelGGL(file_bounds_kernel, dim3((F + 256) / 256), dim3(256), 0, ctx->stream, file, n_items, F,
+                       (const uint64_t *)off, d_fpos, d_fbyte);
     MP_HIP(hipGetLastError());
-    std::vector<uint64_t> fpos(F + 1);
+    std::vector<uint64_t> fpos(F + 1), fbyte(F + 1);
     MP_HIP(hipMemcpyAsync(fpos.data(), d_fpos, sizeof(uint64_t) * (F + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(fbyte.data(), d_fbyte, sizeof(uint64_t) * (F + 1), hipMemcpyDeviceToHost, ctx->stream));
     uint64_t total = 0;
     MP_HIP(hipMemcpyAsync(&total, off + n_items, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<uint64_t> fbyte(F + 1);
-    for (int f = 0; f <= F; f++) {
-        uint64_t v = 0;
-        MP_HIP(hipMemcpyAsync(&v, off + fpos[f], sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-        MP_HIP(hipStreamSynchronize(ctx->stream));
-        fbyte[f] = v;
-    }
     MP_TRY(ensure(ctx, ctx->out, total + 64));
     const uint64_t real_items = fpos[F];
     if (real_items) {
@@ -234,9 +261,10 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     std::vector<char> host(total ? total : 1);
     if (total) MP_HIP(hipMemcpy(host.data(), ctx->out.p, total, hipMemcpyDeviceToHost));
     for (int f = 0; f < F; f++) {
-        const int p = f / T, t = f % T;
-        std::string name = std::string(out_prefix) + "/Rank0/FromTid" + std::to_string(t) + "_ToRank" +
-                           std::to_string(p) + ".paired.fastq";
+        const int smp = f / (MP_NPART * T), p = (f / T) % MP_NPART, t = f % T;
+        std::string dir = std::string(out_prefix) + (batch ? "/sample" + std::to_string(smp) : std::string()) + "/Rank0";
+        if (batch && p == 0 && make_dirs(dir)) return ctx->fail(MP_EIO, "cannot create %s", dir.c_str());
+        std::string name = dir + "/FromTid" + std::to_string(t) + "_ToRank" + std::to_string(p) + ".paired.fastq";
         FILE *fp = fopen(name.c_str(), "wb");
         if (!fp) return ctx->fail(MP_EIO, "cannot create %s", name.c_str());
         const uint64_t a = fbyte[f], b = fbyte[f + 1];

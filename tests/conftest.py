@@ -22,3 +22,17 @@ def _require_build():
                            "metapart_amd/bin/mp_synth") if not _built(p)]
     if missing:
         pytest.fail("not built (python -c 'import __graft_entry__ as g; g.build()'): %s" % ", ".join(missing))
+
+
+def pytest_sessionstart(session):
+    # A GPU test process holds two HIP runtimes: the one bundled with torch and
+    # the system one the library links.  Some GPU tests hand torch device
+    # buffers to the library, which works when torch's runtime starts first.
+    if "not gpu" in (session.config.getoption("markexpr") or ""):
+        return
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()

@@ -84,3 +84,22 @@ def test_plan_ranges_rejects_degenerate():
     with pytest.raises(mp.MPError) as e:
         mp.plan_ranges(H, 1, 4)
     assert e.value.code == -3
+
+
+def test_single_scan_plan_covers_every_tuple():
+    """With one scan and one segment the plan is [0, last non-empty bucket + 1):
+    a batch that enumerates every bucket of a sample sees exactly the tuples of
+    that sample's plain run."""
+    rng = np.random.default_rng(5)
+    for trial in range(8):
+        counts = (rng.random(2 ** 20) < rng.uniform(0.001, 0.9)).astype(np.uint64) * rng.integers(1, 9, 2 ** 20).astype(np.uint64)
+        if trial % 2:
+            counts[-rng.integers(1, 5000):] = 0
+        if counts.sum() == 0:
+            counts[rng.integers(0, 2 ** 20)] = 1
+        H = np.zeros(2 ** 20 + 1, dtype=np.uint64)
+        H[1:] = np.cumsum(counts)
+        scan, thr = mp.plan_ranges(H, 1, 1)
+        last = int(np.nonzero(counts)[0][-1])
+        assert int(scan[0]) == 0 and int(scan[1]) == last + 1
+        assert list(thr) == list(scan)

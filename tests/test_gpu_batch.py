@@ -1,0 +1,114 @@
+"""Batched form: a batch of independent samples partitioned in one launch
+sequence gives, for every sample, the labels, plan and output files of a plain
+run of that sample alone (one chunk, one scan, one segment)."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import metapart_amd as mp
+
+K = 31
+
+
+def _samples():
+    # uneven sizes, own genome pools, different read lengths and error rates
+    specs = [dict(n_reads=2000, n_genomes=3, genome_len_min=4000, genome_len_max=8000, read_len=100, err_rate=0.005),
+             dict(n_reads=30000, n_genomes=12, genome_len_min=5000, genome_len_max=20000, read_len=150, err_rate=0.01),
+             dict(n_reads=8000, n_genomes=40, genome_len_min=4000, genome_len_max=8000, read_len=120, err_rate=0.02),
+             dict(n_reads=2, n_genomes=1, genome_len_min=3000, genome_len_max=3000, read_len=100, err_rate=0.0),
+             dict(n_reads=12000, n_genomes=6, genome_len_min=20000, genome_len_max=40000, read_len_min=60,
+                  read_len_max=150, err_rate=0.01, n_rate=0.002),
+             dict(n_reads=20000, n_genomes=40, genome_len_min=4000, genome_len_max=8000, abund_sigma=1.0,
+                  read_len=100, err_rate=0.005)]
+    return [mp.synth(0, seed=500 + i, **sp) for i, sp in enumerate(specs)]
+
+
+def _hash_dir(d):
+    out = {}
+    for root, _, files in os.walk(d):
+        for f in files:
+            p = os.path.join(root, f)
+            out[os.path.relpath(p, d)] = hashlib.sha256(open(p, "rb").read()).hexdigest()
+    return out
+
+
+def _solo(buf, out_dir):
+    with mp.Context(k=K) as ctx:
+        ctx.load(buf, [0, buf.nbytes])
+        os.makedirs(os.path.join(out_dir, "Rank0"), exist_ok=True)
+        r = mp.partition(ctx, S=1, T=1, out_prefix=out_dir, want_parent=True)
+    return r
+
+
+@pytest.mark.gpu
+def test_batch_equals_solo_runs(tmp_path):
+    print("gpu batch: 6 samples vs solo runs", flush=True)
+    bufs = _samples()
+    solo = []
+    for i, b in enumerate(bufs):
+        solo.append(_solo(b, str(tmp_path / ("solo%d" % i))))
+    with mp.Context(k=K) as ctx:
+        ctx.batch_load(bufs)
+        res, st, fs = ctx.batch_partition()
+        assert st.tuples == sum(int(s.scans[0]["tuples"]) for s in solo)
+        nb = ctx.batch_write(str(tmp_path / "batch"))
+        for i, (b, s) in enumerate(zip(bufs, solo)):
+            assert res[i]["reads"] == len(s.parent)
+            assert res[i]["tuples"] == s.scans[0]["tuples"]
+            assert np.array_equal(res[i]["W"], s.W), i
+            assert np.array_equal(ctx.batch_parent(i).astype(np.uint64), s.parent), i
+            assert res[i]["components"] == s.finish["components"]
+            assert res[i]["largest"] == s.finish["largest"]
+            got = _hash_dir(str(tmp_path / "batch" / ("sample%d" % i)))
+            want = _hash_dir(str(tmp_path / ("solo%d" % i)))
+            assert got == want, i
+        assert nb == sum(s.out_bytes for s in solo)
+        # the same batch again on the same context: identical results
+        res2, _, _ = ctx.batch_partition()
+        for a, b in zip(res, res2):
+            assert np.array_equal(a["W"], b["W"]) and a["components"] == b["components"]
+
+
+@pytest.mark.gpu
+def test_batch_from_device_memory_and_single_sample(tmp_path):
+    print("gpu batch: device-resident input", flush=True)
+    import torch
+    bufs = _samples()[:3]
+    host = np.concatenate([b.view() for b in bufs])
+    ofs = np.concatenate([[0], np.cumsum([b.nbytes for b in bufs])]).astype(np.uint64)
+    d = torch.empty(host.nbytes + 64, dtype=torch.uint8, device="cuda")
+    d[: host.nbytes].copy_(torch.from_numpy(host))
+    torch.cuda.synchronize()
+    with mp.Context(k=K) as ctx:
+        ctx.batch_load_device(d.data_ptr(), host.nbytes, ofs)
+        res, _, _ = ctx.batch_partition()
+        parents = [ctx.batch_parent(i) for i in range(3)]
+    with mp.Context(k=K) as ctx:
+        ctx.batch_load([bufs[1]])
+        one, _, _ = ctx.batch_partition()
+        assert np.array_equal(one[0]["W"], res[1]["W"])
+        assert np.array_equal(ctx.batch_parent(0), parents[1])
+
+
+@pytest.mark.gpu
+def test_batch_rejects_bad_samples():
+    print("gpu batch: error paths", flush=True)
+    good = mp.synth(0, n_reads=2000).tobytes()
+    rec = good.split(b"\n@")
+    odd = b"\n@".join(rec[:-1]) + b"\n"
+    with mp.Context(k=K) as ctx:
+        with pytest.raises(mp.MPError) as e:
+            ctx.batch_load([good, odd])
+        assert e.value.code == -1
+        with pytest.raises(mp.MPError) as e:
+            ctx.batch_load([good] * (mp.MAX_SAMPLES + 1))
+        assert e.value.code == -1
+        with pytest.raises(mp.MPError) as e:
+            ctx.batch_write(None)
+        assert e.value.code == -7
+        ctx.batch_load([good, good])
+        res, _, _ = ctx.batch_partition()
+        assert np.array_equal(res[0]["W"], res[1]["W"])
+        assert np.array_equal(ctx.batch_parent(0), ctx.batch_parent(1))
