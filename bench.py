@@ -11,7 +11,16 @@ range plan, k-mer tuple enumeration, radix sort, frequency filter, union-find,
 compress, pair step, partition plan and the device-side build of the 16
 partitioned FASTQ streams.  Host file writes are not part of the step.
 
+The same line also reports the 3-minimizer variant on the same reads
+("variants") and the batched form ("batched"): a 64-sample mix (seeded genome
+pool per sample of 20-300 genomes of 0.1-1 Mb, log-uniform 10k-5M reads of
+100-150 bp, 0.5-2 % errors, k = 31) resident in HBM, run once through the
+plain per-sample path (one context, samples one after another, each a full
+histogram/plan/scan/finish/writer run) and once as one batch; the plans of
+every sample must agree between the two.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--reads N] [--no-cpu-baseline]
+                       [--no-min3] [--no-batch] [--batch-samples 64]
 """
 from __future__ import annotations
 
@@ -87,7 +96,13 @@ def main():
     ap.add_argument("--chunks", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=600_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--min3", action="store_true", help="also time the 3-minimizer variant")
+    ap.add_argument("--no-min3", action="store_true", help="skip the 3-minimizer variant")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-sample mix")
+    ap.add_argument("--batch-samples", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=60.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -132,7 +147,7 @@ def main():
         return r, t
 
     results = {}
-    for variant in (["plain", "min3"] if args.min3 else ["plain"]):
+    for variant in (["plain"] if args.no_min3 else ["plain", "min3"]):
         ctx = mp.Context(k=args.k, min3=(variant == "min3"))
         for _ in range(args.warmup):
             step(ctx)
@@ -190,9 +205,155 @@ def main():
         "generate_s": gen_s,
     }
     if "min3" in results:
-        out["variants"] = {"min3": {"reads_per_s": results["min3"]["reads_per_s"],
-                                    "ms_per_step": results["min3"]["ms_per_step"]}}
+        m3 = results["min3"]
+        out["variants"] = {"min3": {"reads_per_s": m3["reads_per_s"], "ms_per_step": m3["ms_per_step"],
+                                    "tuples": int(m3["scan"]["tuples"]), "finish": m3["finish"],
+                                    "host_stage_s": m3["host_stage_s"],
+                                    "stages_ms": {k: m3["scan"][k] for k in ("ms_enumerate", "ms_sort", "ms_filter",
+                                                                             "ms_union")}}}
+    if not args.no_batch:
+        del d_fq
+        buf.free()
+        torch.cuda.empty_cache()
+        out["batched"] = batch_section(args, mp, torch, dev)
+    if not args.no_serve:
+        out["served"] = served_section(args, mp)
     print(json.dumps(out))
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if xs else 0.0
+
+
+def served_section(args, mp):
+    """Poisson-arriving jobs from host memory: the request loop (2 engines,
+    continuous batching under an HBM budget) against the plain per-job path
+    (one context, jobs one at a time in arrival order), same arrivals."""
+    import numpy as np
+    params = mp.job_params(n_jobs=args.serve_jobs, seed=7)
+    t0 = time.time()
+    bufs = [mp.synth(0, **p) for p in params]
+    gen_s = time.time() - t0
+    reads = [p["n_reads"] for p in params]
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(bufs)))
+    warm = mp.synth(0, n_reads=20000, seed=5, read_len=150)
+
+    # plain per-job path
+    ctx = mp.Context(k=args.k)
+    ctx.load(warm, [0, warm.nbytes])
+    mp.partition(ctx, S=1, T=1, build_output=False)
+    lat_plain = []
+    start = time.perf_counter()
+    for i, b in enumerate(bufs):
+        now = time.perf_counter() - start
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        ctx.load(b, [0, b.nbytes])
+        mp.partition(ctx, S=1, T=1, build_output=False)
+        lat_plain.append(time.perf_counter() - start - arrivals[i])
+    plain_span = time.perf_counter() - start
+    ctx.close()
+
+    # served
+    srv = mp.Server(k=args.k, engines=2)
+    srv.wait(srv.submit(warm))
+    jobs = []
+    start = time.perf_counter()
+    for i, b in enumerate(bufs):
+        now = time.perf_counter() - start
+        if now < arrivals[i]:
+            time.sleep(arrivals[i] - now)
+        jobs.append(srv.submit(b))
+    res = [srv.wait(j)[0] for j in jobs]
+    srv.close()
+    lat = [r["t_done"] - r["t_submit"] for r in res]
+    span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    for b in bufs:
+        b.free()
+    return {"workload": "%d jobs, log-uniform 50k-2M SE reads of 150 bp, 1%% error, k=%d, fresh genome pool per job "
+                        "(20-100 genomes of 0.1-1 Mb), FASTQ in host memory, Poisson arrivals at %.0f jobs/s"
+                        % (len(bufs), args.k, args.serve_rate),
+            "jobs": len(bufs), "reads": int(sum(reads)), "rate_jobs_per_s": args.serve_rate,
+            "served": {"p50_ms": 1e3 * _pct(lat, 50), "p99_ms": 1e3 * _pct(lat, 99),
+                       "reads_per_s": sum(reads) / span, "engines": 2,
+                       "mean_batch_jobs": float(np.mean([r["batch_jobs"] for r in res]))},
+            "plain": {"p50_ms": 1e3 * _pct(lat_plain, 50), "p99_ms": 1e3 * _pct(lat_plain, 99),
+                      "reads_per_s": sum(reads) / plain_span},
+            "generate_s": gen_s}
+
+
+def batch_section(args, mp, torch, dev):
+    """Plain per-sample path vs one batch over the same HBM-resident mix."""
+    import numpy as np
+    params = mp.mix_params(n_samples=args.batch_samples, seed=4)
+    t0 = time.time()
+    bufs = [mp.synth(0, **p) for p in params]
+    gen_s = time.time() - t0
+    sizes = [b.nbytes for b in bufs]
+    ofs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    total = int(ofs[-1])
+    d = torch.empty(total + 64, dtype=torch.uint8, device=dev)
+    # the plain path takes each sample as its own 16-byte aligned buffer
+    aofs = np.concatenate([[0], np.cumsum([(n + 255) // 256 * 256 for n in sizes])]).astype(np.uint64)
+    da = torch.empty(int(aofs[-1]) + 64, dtype=torch.uint8, device=dev)
+    for b, o, a in zip(bufs, ofs[:-1], aofs[:-1]):
+        h = torch.from_numpy(b.view())
+        d[int(o): int(o) + b.nbytes].copy_(h)
+        da[int(a): int(a) + b.nbytes].copy_(h)
+    torch.cuda.synchronize()
+    for b in bufs:
+        b.free()
+    reads = sum(p["n_reads"] for p in params)
+    base = d.data_ptr()
+
+    ctx = mp.Context(k=args.k)
+
+    def plain_step():
+        Ws = []
+        for i in range(len(sizes)):
+            ctx.load_device(da.data_ptr() + int(aofs[i]), sizes[i], [0, sizes[i]])
+            Ws.append(mp.partition(ctx, S=1, T=1, out_prefix=None).W)
+        return Ws
+
+    plain_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        Wp = plain_step()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t0) / args.batch_steps
+    ctx.close()
+    del da
+    torch.cuda.empty_cache()
+
+    bctx = mp.Context(k=args.k)
+
+    def batch_step():
+        bctx.batch_load_device(base, total, ofs)
+        res, st, fs = bctx.batch_partition()
+        nb = bctx.batch_write(None)
+        return res, st, fs, nb
+
+    batch_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        res, st, fs, nb = batch_step()
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - t0) / args.batch_steps
+    bctx.close()
+    equal = all(np.array_equal(w, r["W"]) for w, r in zip(Wp, res))
+    return {"workload": "%d samples, own genome pools (20-300 genomes of 0.1-1 Mb), log-uniform 10k-5M SE reads of "
+                        "100-150 bp, 0.5-2%% error, k=%d, one chunk/scan/segment per sample" % (len(sizes), args.k),
+            "samples": len(sizes), "reads": reads, "tuples": int(st.tuples),
+            "plain_reads_per_s": reads / plain_s, "batched_reads_per_s": reads / batch_s,
+            "speedup": plain_s / batch_s, "ms_plain": 1e3 * plain_s, "ms_batched": 1e3 * batch_s,
+            "plans_equal": bool(equal), "batch_stages_ms": {k: getattr(st, k) for k in
+                                                            ("ms_enumerate", "ms_sort", "ms_filter", "ms_union")},
+            "batch_sort_passes": st.sort_passes, "batch_sort_pass_ms": st.ms_sort_pass_avg,
+            "generate_s": gen_s}
 
 
 if __name__ == "__main__":

@@ -209,6 +209,51 @@ int mp_batch_get_parent(mp_ctx *ctx, int s, uint32_t *parent);
  * sample (out_dir NULL: build the streams on the device only). */
 int mp_batch_write(mp_ctx *ctx, const char *out_dir, uint64_t *bytes_out);
 
+/* ---------------------------------------------------------------- served form
+ * A request loop inside the library (no counterpart in the reference, which
+ * runs one sample per process; these entry points serve callers that submit
+ * partition jobs over time).  Jobs queue as they arrive; `engines` contexts,
+ * each with its own stream and an equal share of hbm_budget, run on host
+ * threads.  An idle engine takes, in arrival order, every queued job whose
+ * estimated device footprint fits its share (at most max_jobs_per_batch) and
+ * partitions them as one batch, so jobs arriving while others run are admitted
+ * as engines and memory free up, and batches overlap on the device.  A job's
+ * result equals a plain run of its sample alone (see the batched form). */
+typedef struct mp_server mp_server;
+
+typedef struct {
+    int device;
+    int k;
+    int engines;              /* concurrent contexts (0: 2) */
+    int max_jobs_per_batch;   /* 0: 64; at most MP_MAX_SAMPLES */
+    uint64_t hbm_budget;      /* device bytes for all engines (0: 3/4 of free memory) */
+} mp_server_cfg;
+
+typedef struct {
+    uint64_t job;
+    int status;               /* 0 or the MP_E* code of the failed job */
+    int batch_jobs;           /* jobs in the batch that ran it */
+    uint64_t reads, tuples, components, largest;
+    uint64_t W[MP_NPART + 1];
+    double t_submit, t_start, t_done;  /* seconds on the server clock (mp_server_clock) */
+} mp_job_result;
+
+int mp_server_create(const mp_server_cfg *cfg, mp_server **out);
+
+/* Queues one single-end FASTQ sample (even, non-zero record count).  fq must
+ * stay valid until mp_server_wait returns for the job. */
+int mp_server_submit(mp_server *srv, const char *fq, uint64_t nbytes, int want_parent, uint64_t *job);
+
+/* Blocks until the job is done; parent (cap entries, may be NULL) receives
+ * its labels over its own read ids if it was submitted with want_parent.
+ * A job is waited for once. */
+int mp_server_wait(mp_server *srv, uint64_t job, mp_job_result *res, uint32_t *parent, uint64_t cap);
+
+double mp_server_clock(const mp_server *srv);
+
+/* Runs every queued job to completion, then frees the engines. */
+int mp_server_destroy(mp_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

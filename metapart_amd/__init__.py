@@ -65,6 +65,23 @@ class SampleResult(ct.Structure):
                 "W": np.array(self.W[:], dtype=np.uint64)}
 
 
+class ServerCfg(ct.Structure):
+    _fields_ = [("device", ct.c_int), ("k", ct.c_int), ("engines", ct.c_int), ("max_jobs_per_batch", ct.c_int),
+                ("hbm_budget", ct.c_uint64)]
+
+
+class JobResult(ct.Structure):
+    _fields_ = [("job", ct.c_uint64), ("status", ct.c_int), ("batch_jobs", ct.c_int), ("reads", ct.c_uint64),
+                ("tuples", ct.c_uint64), ("components", ct.c_uint64), ("largest", ct.c_uint64),
+                ("W", ct.c_uint64 * (NPART + 1)), ("t_submit", ct.c_double), ("t_start", ct.c_double),
+                ("t_done", ct.c_double)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "W"}
+        d["W"] = np.array(self.W[:], dtype=np.uint64)
+        return d
+
+
 _lib = None
 _synth = None
 P64 = ct.POINTER(ct.c_uint64)
@@ -103,6 +120,11 @@ def lib():
             "mp_batch_partition": (i, [vp, ct.POINTER(SampleResult), ct.POINTER(ScanStats), ct.POINTER(FinishStats)]),
             "mp_batch_get_parent": (i, [vp, i, vp]),
             "mp_batch_write": (i, [vp, cp, P64]),
+            "mp_server_create": (i, [ct.POINTER(ServerCfg), ct.POINTER(vp)]),
+            "mp_server_submit": (i, [vp, vp, u64, i, P64]),
+            "mp_server_wait": (i, [vp, u64, ct.POINTER(JobResult), vp, u64]),
+            "mp_server_clock": (ct.c_double, [vp]),
+            "mp_server_destroy": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -117,7 +139,8 @@ SYMBOLS = ["mp_index_fastq", "mp_index_buffer", "mp_plan_ranges", "mp_ctx_create
            "mp_last_error", "mp_ctx_device_bytes", "mp_count_chunks", "mp_count_loaded", "mp_chunk_first_ids",
            "mp_load_reads", "mp_load_reads_device", "mp_scan", "mp_finish", "mp_get_parent", "mp_get_bad", "mp_dump_tuples",
            "mp_write_partitions", "mp_batch_load", "mp_batch_load_device", "mp_batch_partition", "mp_batch_get_parent",
-           "mp_batch_write"]
+           "mp_batch_write", "mp_server_create", "mp_server_submit", "mp_server_wait", "mp_server_clock",
+           "mp_server_destroy"]
 
 
 def _u64(a) -> np.ndarray:
@@ -317,6 +340,64 @@ class Context:
         return int(nb.value)
 
 
+class Server:
+    """The served form (mp_server): submit jobs at any time, wait for each."""
+
+    def __init__(self, k: int, engines: int = 2, max_jobs_per_batch: int = 64, hbm_budget: int = 0, device: int = 0):
+        cfg = ServerCfg(device=device, k=k, engines=engines, max_jobs_per_batch=max_jobs_per_batch,
+                        hbm_budget=hbm_budget)
+        h = ct.c_void_p()
+        rc = lib().mp_server_create(ct.byref(cfg), ct.byref(h))
+        if rc:
+            raise MPError(rc, "mp_server_create")
+        self._h = h
+        self._keep = {}  # job -> (buffer, reads hint): the bytes stay alive until waited for
+
+    def submit(self, buf, want_parent: bool = False) -> int:
+        ptr, n = _buf_ptr(buf)
+        job = ct.c_uint64(0)
+        rc = lib().mp_server_submit(self._h, ct.c_void_p(ptr), n, 1 if want_parent else 0, ct.byref(job))
+        if rc:
+            raise MPError(rc, "mp_server_submit")
+        self._keep[job.value] = (buf, n)
+        return job.value
+
+    def wait(self, job: int, want_parent: bool = False):
+        """(result dict, parent labels or None).  A failed job raises MPError."""
+        res = JobResult()
+        _, n = self._keep.get(job, (None, 0))
+        par = np.zeros(n // 6 + 1, dtype=np.uint32) if want_parent else None  # a record is >= 6 bytes
+        rc = lib().mp_server_wait(self._h, job, ct.byref(res), par.ctypes.data if par is not None else None,
+                                  par.size if par is not None else 0)
+        self._keep.pop(job, None)
+        if rc:
+            raise MPError(rc, "mp_server_wait(%d)" % job)
+        d = res.as_dict()
+        if d["status"]:
+            raise MPError(d["status"], "job %d failed" % job)
+        return d, (par[: d["reads"]].copy() if par is not None else None)
+
+    def clock(self) -> float:
+        return float(lib().mp_server_clock(self._h))
+
+    def close(self):
+        if self._h:
+            lib().mp_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class RunResult:
     W: np.ndarray
@@ -421,6 +502,22 @@ def mix_params(n_samples: int = 64, seed: int = 4, reads_min: int = 10_000, read
                         genome_len_min=genome_len_min, genome_len_max=genome_len_max, abund_sigma=1.0,
                         n_reads=n, read_len=int(rng.integers(len_min, len_max + 1)),
                         err_rate=float(rng.uniform(err_min, err_max))))
+    return out
+
+
+def job_params(n_jobs: int = 64, seed: int = 7, reads_min: int = 50_000, reads_max: int = 2_000_000,
+               genomes_min: int = 20, genomes_max: int = 100, genome_len_min: int = 100_000,
+               genome_len_max: int = 1_000_000):
+    """Partition jobs of the served workload: log-uniform read counts (even),
+    150 bp reads, 1 % errors, every job with a fresh seeded genome pool."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_jobs):
+        n = int(np.exp(rng.uniform(np.log(reads_min), np.log(reads_max))))
+        n += n & 1
+        out.append(dict(seed=7919 * (seed + 1) + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
+                        genome_len_min=genome_len_min, genome_len_max=genome_len_max, abund_sigma=1.0,
+                        n_reads=n, read_len=150, err_rate=0.01))
     return out
 
 

@@ -33,7 +33,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SRCS = ["mp_util.hip", "mp_parse.hip", "mp_kmers.hip", "mp_sort.hip", "mp_graph.hip", "mp_writer.hip",
             "mp_api.hip"]
-HOST_SRCS = ["mp_host.cpp"]
+HOST_SRCS = ["mp_host.cpp", "mp_server.cpp"]
 
 
 def _run(cmd, cwd=None):
@@ -51,13 +51,14 @@ def build_lib(jobs: int = 8) -> str:
         jobs_list.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
     for s in HOST_SRCS:
         o = os.path.join(BUILD, s.replace(".cpp", ".o"))
-        jobs_list.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, s), "-o", o])
+        jobs_list.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", "-I/opt/rocm/include",
+                          "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, s), "-o", o])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, jobs_list))
     objs = [os.path.join(BUILD, s.replace(".hip", ".o")) for s in HIP_SRCS] + \
            [os.path.join(BUILD, s.replace(".cpp", ".o")) for s in HOST_SRCS]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

@@ -19,7 +19,7 @@
 //
 // 3-minimizer variant (countMmers_3minimizers.c:83-271,
 // enumerateKmers_3minimizers.c:129-474): a sequential window state machine per
-// read, one lane per read, restating the reference loop on the raw bytes.
+// read, one lane per read, over the same O(1) canonical k-mers.
 #include "mp_internal.h"
 
 #include <algorithm>
@@ -387,110 +387,98 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
 }
 
 // ---------------------------------------------------------------- 3-minimizer
-// Rolling codes exactly as the reference (countMmers_3minimizers.c:107-190).
-struct Roll {
-    uint64_t F = 0, R = 0, mask = 0;
-    int lastN = -1;
-};
+// The reference rolls forward and reverse codes through the read
+// (countMmers_3minimizers.c:107-190); position i's canonical k-mer and its
+// 'N' test are exactly kmer_at(), which reads the packed streams (8x smaller
+// than the FASTQ bytes, so a lane walking its read stays in cache).  The
+// window state machine itself is sequential per read and is restated as is.
 
 // counter: countMmers_3minimizers.c:83-271 (no range filter)
-template <typename Sink>
-__device__ void min3_count_read(const uint8_t *s, int L, int K, int bshift, Sink sink)
+template <typename Src, typename Sink>
+__device__ void min3_count_read(int L, int K, int bshift, Src kmer, Sink sink)
 {
     if (L <= K) return;
     const int last = L - K;
-    const uint64_t mask = 2 * K >= 64 ? 0ull : (1ull << (2 * K)) - 1;
-    uint64_t F = 0, R = 0, minK = 0, minB = 0;
-    int lastN = -1, next = last / 3;
-    for (int j = 0; j < K; j++) {
-        const uint8_t c = s[j];
-        if (c == 'N') lastN = j;
-        R |= (uint64_t)(3 - code_of(c)) << (2 * j);
-        F = (F << 2) | code_of(c);
-    }
-    if (lastN == -1) {
-        const uint64_t c = F < R ? F : R;
+    uint64_t minK = 0, minB = 0, c;
+    bool fwd;
+    int next = last / 3;
+    if (kmer(0, c, fwd)) {
         minK = c;
         minB = c >> bshift;
     }
     for (int i = 1; i <= last; i++) {
-        const uint8_t ch = s[i + K - 1];
-        R = (R >> 2) | ((uint64_t)(3 - code_of(ch)) << (2 * (K - 1)));
-        F = ((F << 2) | code_of(ch)) & mask;
-        if (ch == 'N') lastN = i + K - 1;
-        if (i > lastN) {
-            const uint64_t c = F < R ? F : R;
-            const uint64_t b = c >> bshift;
-            if (i > next) {
-                if (minK != 0) sink(minB);
-                minK = c;
-                minB = b;
-                next += last / 3;
-                if (next + last / 3 > last) next = last;
-            } else if (c < minK || minK == 0) {
-                minK = c;
-                minB = b;
-            }
+        if (!kmer(i, c, fwd)) continue;  // an 'N' in the window (i <= lastN)
+        const uint64_t b = c >> bshift;
+        if (i > next) {
+            if (minK != 0) sink(minB);
+            minK = c;
+            minB = b;
+            next += last / 3;
+            if (next + last / 3 > last) next = last;
+        } else if (c < minK || minK == 0) {
+            minK = c;
+            minB = b;
         }
     }
     if (minK != 0) sink(minB);
 }
 
-// enumerator: enumerateKmers_3minimizers.c:129-474 (range filter before the window logic)
-template <typename Sink>
-__device__ void min3_enum_read(const uint8_t *s, int L, int K, int bshift, uint64_t lo, uint64_t hi, Sink sink)
+// enumerator: enumerateKmers_3minimizers.c:129-474 (range filter before the
+// window logic).  The reference keeps the minimum's neighbour characters,
+// complemented when the minimum is a reverse k-mer; here the minimum's
+// position and strand are kept and the characters read when it is emitted.
+// An emission inside the loop orders the pair by the strand of the k-mer that
+// closes the window, the final one always as (next, prev), as the reference.
+template <typename Src, typename Sink>
+__device__ void min3_enum_read(const uint8_t *s, int L, int K, int bshift, uint64_t lo, uint64_t hi, Src kmer,
+                               Sink sink)
 {
     if (L <= K) return;
     const int last = L - K;
-    const uint64_t mask = 2 * K >= 64 ? 0ull : (1ull << (2 * K)) - 1;
-    uint64_t F = 0, R = 0, minK = 0;
-    uint8_t minPrev = 'N', minNext = 'N';
-    int lastN = -1;
-    uint8_t prev = 'N', next = s[K];
-    for (int j = 0; j < K; j++) {
-        const uint8_t c = s[j];
-        if (c == 'N') lastN = j;
-        R |= (uint64_t)(3 - code_of(c)) << (2 * j);
-        F = (F << 2) | code_of(c);
-    }
-    if (lastN == -1) {
-        const bool fwd = F < R;
-        const uint64_t c = fwd ? F : R;
+    uint64_t minK = 0, c;
+    int minPos = 0;
+    bool minFwd = true, fwd;
+    auto neighbours = [&](uint8_t &cp, uint8_t &cn) {
+        const uint8_t p = minPos == 0 ? (uint8_t)'N' : s[minPos - 1];
+        const uint8_t n = minPos == last ? (uint8_t)'N' : s[minPos + K];
+        cp = minFwd ? p : rc_char(p);
+        cn = minFwd ? n : rc_char(n);
+    };
+    if (kmer(0, c, fwd)) {
         const uint64_t b = c >> bshift;
         if (b >= lo && b < hi) {
             minK = c;
-            minPrev = fwd ? prev : rc_char(prev);
-            minNext = fwd ? next : rc_char(next);
+            minPos = 0;
+            minFwd = fwd;
         }
     }
     int nextW = last / 3;
     for (int i = 1; i <= last; i++) {
-        prev = s[i - 1];
-        next = i == last ? (uint8_t)'N' : s[i + K];
-        const uint8_t ch = s[i + K - 1];
-        R = (R >> 2) | ((uint64_t)(3 - code_of(ch)) << (2 * (K - 1)));
-        F = ((F << 2) | code_of(ch)) & mask;
-        if (ch == 'N') lastN = i + K - 1;
-        if (i <= lastN) continue;
-        const bool fwd = F < R;
-        const uint64_t c = fwd ? F : R;
+        if (!kmer(i, c, fwd)) continue;
         const uint64_t b = c >> bshift;
         if (!(b >= lo && b < hi)) continue;
-        const uint8_t cp = fwd ? prev : rc_char(prev), cn = fwd ? next : rc_char(next);
         if (i > nextW) {
-            if (minK != 0) sink(minK, fwd ? adj_of(minPrev, minNext) : adj_of(minNext, minPrev));
+            if (minK != 0) {
+                uint8_t cp, cn;
+                neighbours(cp, cn);
+                sink(minK, fwd ? adj_of(cp, cn) : adj_of(cn, cp));
+            }
             minK = c;
-            minPrev = cp;
-            minNext = cn;
+            minPos = i;
+            minFwd = fwd;
             nextW += last / 3;
             if (nextW + last / 3 > last) nextW = last;
         } else if (c < minK || minK == 0) {
             minK = c;
-            minPrev = cp;
-            minNext = cn;
+            minPos = i;
+            minFwd = fwd;
         }
     }
-    if (minK != 0) sink(minK, adj_of(minNext, minPrev));
+    if (minK != 0) {
+        uint8_t cp, cn;
+        neighbours(cp, cn);
+        sink(minK, adj_of(cn, cp));
+    }
 }
 
 template <int MODE>
@@ -503,10 +491,13 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
     uint32_t cnt = 0;
     const uint8_t *s = nullptr;
     int L = 0;
+    uint64_t b0 = 0;
     if (r < P.N) {
         s = P.fq + P.seq_off[r];
         L = (int)P.seq_len[r];
+        b0 = P.base_off[r];
     }
+    auto kmer = [&](int i, uint64_t &c, bool &fwd) { return kmer_at(P, b0 + (uint64_t)i, (uint32_t)i, c, fwd); };
     if (MODE == kModeHist) {
         if (r < P.N) {
             int lo = 0, hi = P.C - 1;
@@ -515,11 +506,11 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
                 if (P.chunk_ids[mid] <= r) lo = mid; else hi = mid - 1;
             }
             uint32_t *cnts = P.counts + (size_t)lo * (MP_NBUCKETS + 2);
-            min3_count_read(s, L, P.K, P.bshift, [&](uint64_t b) { atomicAdd(&cnts[b], 1u); });
+            min3_count_read(L, P.K, P.bshift, kmer, [&](uint64_t b) { atomicAdd(&cnts[b], 1u); });
         }
         return;
     }
-    if (r < P.N) min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, [&](uint64_t, uint8_t) { cnt++; });
+    if (r < P.N) min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, kmer, [&](uint64_t, uint8_t) { cnt++; });
     // exclusive scan of cnt over the workgroup
     uint32_t inc = cnt;
     for (int d = 1; d < kWave; d <<= 1) {
@@ -541,7 +532,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
     if (r >= P.N || cnt == 0) return;
     uint64_t pos = P.wg_off[blockIdx.x] + woff + inc - cnt;
     const uint32_t label = P.scan0 ? (uint32_t)r : (P.bad[r] ? (uint32_t)r : P.parent[r]);
-    min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, [&](uint64_t k, uint8_t a) {
+    min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, kmer, [&](uint64_t k, uint8_t a) {
         P.keys[pos] = k;
         P.adj[pos] = a;
         P.lab[pos] = label;
