@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--batch-samples", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--c3", action="store_true",
+                    help="also run the large out-of-core share (100M reads; needs ~45 GB of host memory)")
+    ap.add_argument("--c3-reads", type=int, default=100_000_000)
     ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=60.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--copy-probe", action="store_true",
@@ -218,7 +221,44 @@ def main():
         out["batched"] = batch_section(args, mp, torch, dev)
     if not args.no_serve:
         out["served"] = served_section(args, mp)
+    if args.c3:
+        out["out_of_core"] = c3_section(args, mp, torch)
     print(json.dumps(out))
+
+
+def c3_section(args, mp, torch):
+    """1000 genomes of 0.5-8 Mb (log-normal sigma 1.5), 150 bp reads, 1 % errors,
+    k = 31: the FASTQ comes from host memory through pinned staging, and the
+    number of m-mer bucket passes is the fewest whose tuples fit the HBM budget."""
+    from metapart_amd import tune
+    t0 = time.time()
+    buf = mp.synth(3, n_reads=args.c3_reads)
+    gen_s = time.time() - t0
+    cofs = mp.index_buffer(buf, 64)
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(free * 0.92)
+    ctx = mp.Context(k=args.k, hbm_budget=budget)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.load(buf, cofs)
+    t_load = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    hist, _, kmers, _ = ctx.count()
+    bases = kmers + ctx.N * (args.k - 1)
+    S, need = tune.scans_for_budget(hist, ctx.N, buf.nbytes, bases, budget)
+    r = mp.partition(ctx, S=S, T=1, out_prefix=None, hist=hist)
+    t_part = time.perf_counter() - t1
+    out = {"workload": "%d SE reads x 150 bp, 1000 genomes 0.5-8 Mb, log-normal(1.5), 1%% error, k=%d, 64 chunks, "
+                       "T=1, scans from the HBM budget" % (ctx.N, args.k),
+           "reads": ctx.N, "fastq_bytes": buf.nbytes, "kmers": kmers, "scans": S,
+           "scan_tuples": [int(x["tuples"]) for x in r.scans], "budget_bytes": budget, "planned_bytes": need,
+           "device_bytes": ctx.device_bytes, "ms_load_from_host": 1e3 * t_load, "ms_partition": 1e3 * t_part,
+           "reads_per_s": ctx.N / t_part, "reads_per_s_with_upload": ctx.N / (t_part + t_load),
+           "stages_ms": [{k: x[k] for k in ("ms_enumerate", "ms_sort", "ms_filter", "ms_union")} for x in r.scans],
+           "finish": r.finish, "generate_s": gen_s}
+    ctx.close()
+    buf.free()
+    return out
 
 
 def _pct(xs, q):

@@ -187,10 +187,11 @@ def _buf_ptr(buf):
 class Context:
     """One device context (mp_ctx)."""
 
-    def __init__(self, k: int, min3: bool = False, device: int = 0, hbm_budget: int = 0):
+    def __init__(self, k: int, min3: bool = False, device: int = 0, hbm_budget: int = 0, sort_tile: int = 0):
         self.k = k
         self.min3 = bool(min3)
-        cfg = Cfg(device=device, k=k, min3=1 if min3 else 0, hbm_budget=hbm_budget, sort_digit_bits=0, sort_tile=0)
+        cfg = Cfg(device=device, k=k, min3=1 if min3 else 0, hbm_budget=hbm_budget, sort_digit_bits=0,
+                  sort_tile=sort_tile)
         h = ct.c_void_p()
         rc = lib().mp_ctx_create(ct.byref(cfg), ct.byref(h))
         if rc:
@@ -409,10 +410,12 @@ class RunResult:
 
 
 def partition(ctx: Context, S: int = 1, T: int = 1, out_prefix: Optional[str] = None, want_parent: bool = False,
-              il_start=None, build_output: bool = True) -> RunResult:
-    """The metapart stages on reads already loaded in ctx: histogram, plan, S scans,
-    finish, writer (device streams; files too when out_prefix is given)."""
-    hist, reads, total, _ = ctx.count()
+              il_start=None, build_output: bool = True, hist: Optional[np.ndarray] = None) -> RunResult:
+    """The metapart stages on reads already loaded in ctx: histogram (unless
+    given), plan, S scans, finish, writer (device streams; files too when
+    out_prefix is given)."""
+    if hist is None:
+        hist, reads, total, _ = ctx.count()
     scan, thread = plan_ranges(hist, S, T)
     scans = []
     for s in range(S):

@@ -4,6 +4,9 @@
 
 #include "mp_internal.h"
 
+#include <algorithm>
+#include <thread>
+
 using namespace mp;
 
 extern "C" {
@@ -51,7 +54,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->scanst, &ctx->hpart, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->scanst, &ctx->hpart, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -60,6 +63,10 @@ int mp_ctx_destroy(mp_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->pev)
         if (e) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; b++) {
+        if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
+        if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -91,10 +98,51 @@ static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *c
     return 0;
 }
 
+// Host to device copy of FASTQ bytes.  Large inputs go through two pinned
+// staging buffers: host threads fill one while the DMA engine drains the
+// other, so pageable memory is read once at memory speed and PCIe runs at the
+// pinned rate.
+constexpr uint64_t kStageBytes = 256ull << 20;
+constexpr int kStageThreads = 8;
+
+static void parallel_copy(char *dst, const char *src, uint64_t n)
+{
+    std::vector<std::thread> th;
+    const uint64_t part = (n + kStageThreads - 1) / kStageThreads;
+    for (int t = 0; t < kStageThreads; t++) {
+        const uint64_t a = t * part, b = std::min(n, a + part);
+        if (a >= b) break;
+        th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
+    }
+    for (auto &x : th) x.join();
+}
+
+static int copy_to_device(mp_ctx *ctx, uint8_t *dst, const char *src, uint64_t nbytes)
+{
+    if (nbytes < 2 * kStageBytes) {
+        MP_HIP(hipMemcpyAsync(dst, src, nbytes, hipMemcpyHostToDevice, ctx->stream));
+        return 0;
+    }
+    for (int b = 0; b < 2; b++) {
+        if (!ctx->stage[b]) MP_HIP(hipHostMalloc(&ctx->stage[b], kStageBytes, hipHostMallocDefault));
+        if (!ctx->stage_ev[b]) MP_HIP(hipEventCreateWithFlags(&ctx->stage_ev[b], hipEventDisableTiming));
+    }
+    for (uint64_t off = 0, c = 0; off < nbytes; off += kStageBytes, c++) {
+        const int b = (int)(c & 1);
+        const uint64_t len = std::min(kStageBytes, nbytes - off);
+        if (ctx->stage_used[b]) MP_HIP(hipEventSynchronize(ctx->stage_ev[b]));  // its previous copy has left
+        parallel_copy((char *)ctx->stage[b], src + off, len);
+        MP_HIP(hipMemcpyAsync(dst + off, ctx->stage[b], len, hipMemcpyHostToDevice, ctx->stream));
+        MP_HIP(hipEventRecord(ctx->stage_ev[b], ctx->stream));
+        ctx->stage_used[b] = true;
+    }
+    return 0;
+}
+
 static int upload_fq(mp_ctx *ctx, const char *fq, uint64_t nbytes)
 {
     MP_TRY(ensure(ctx, ctx->fq, nbytes + 64));
-    MP_HIP(hipMemcpyAsync(ctx->fq.p, fq, nbytes, hipMemcpyHostToDevice, ctx->stream));
+    MP_TRY(copy_to_device(ctx, (uint8_t *)ctx->fq.p, fq, nbytes));
     MP_TRY(fill_u32(ctx, (uint8_t *)ctx->fq.p + ((nbytes + 3) & ~3ull), 0, 8));
     ctx->d_fq = (const uint8_t *)ctx->fq.p;
     ctx->fq_owned = true;
@@ -302,8 +350,7 @@ int mp_batch_load(mp_ctx *ctx, int n, const char *const *fq, const uint64_t *nby
     for (int i = 0; i < n; i++) ofs[i + 1] = ofs[i] + nbytes[i];
     MP_TRY(ensure(ctx, ctx->fq, ofs[n] + 64));
     for (int i = 0; i < n; i++)
-        if (nbytes[i])
-            MP_HIP(hipMemcpyAsync((uint8_t *)ctx->fq.p + ofs[i], fq[i], nbytes[i], hipMemcpyHostToDevice, ctx->stream));
+        if (nbytes[i]) MP_TRY(copy_to_device(ctx, (uint8_t *)ctx->fq.p + ofs[i], fq[i], nbytes[i]));
     MP_TRY(fill_u32(ctx, (uint8_t *)ctx->fq.p + ((ofs[n] + 3) & ~3ull), 0, 8));
     ctx->d_fq = (const uint8_t *)ctx->fq.p;
     ctx->fq_owned = true;

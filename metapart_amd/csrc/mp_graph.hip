@@ -6,10 +6,11 @@
 // it is a root, and findRoot stops at a node whose parent is BAD, which is the
 // same as stopping at a self-parent (unionfind.h:19-55).
 //
-// Filter (markMischiefKmers, metapart.c:2270-2519): one lane per tuple; the
-// lane holding a run's first tuple evaluates the run unless it is the last run
-// of its segment.  Zeroing is recorded in a bitmap and applied afterwards, so
-// no lane sees a zeroed key while another lane is still finding run starts.
+// Filter (markMischiefKmers, metapart.c:2270-2519): tiles of tuples staged in
+// LDS; the lane holding a run's first tuple evaluates the run unless it is
+// the last run of its segment.  Zeroing is recorded in a bitmap and applied
+// afterwards, so no lane sees a zeroed key while another lane is still
+// finding run starts.
 //
 // Union (unionFindPar2 + unionFind, metapart.c:2520-2800): the reference
 // star-unions the non-BAD labels of each run and always hooks the smaller root
@@ -65,20 +66,15 @@ __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, in
     pos[t] = lo;
 }
 
-// Evaluates the run starting at tuple i (if i starts a run).  Returns 0 (not
-// evaluated), 1 (evaluated) or 2 (evaluated and zeroed).
-__device__ int filter_run(const GParams &G, uint64_t i)
+// Evaluates run [i, j) (not the last run of its segment): the in/out
+// neighbour classes of its tuples decide whether the k-mer is zeroed (both
+// sides branch) or which labels are filtered.  Returns 1, or 2 when zeroed.
+template <typename AdjAt>
+__device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at)
 {
-    uint64_t s0, s1;
-    seg_of(G, i, s0, s1);
-    const uint64_t k = G.keys[i];
-    if (i != s0 && G.keys[i - 1] == k) return 0;  // not a run start
-    uint64_t j = i + 1;
-    while (j < s1 && G.keys[j] == k) j++;
-    if (j == s1) return 0;  // the last run of a segment is never evaluated
     uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint64_t t = i; t < j; t++) {
-        const uint8_t a = G.adj[t];
+        const uint8_t a = adj_at(t);
         in[a & 7]++;
         out[(a >> 3) & 7]++;
     }
@@ -113,7 +109,7 @@ __device__ int filter_run(const GParams &G, uint64_t i)
     const float deg = (float)(int)(j - i);
     const bool ratio = (double)__fdiv_rn((float)sIn, deg) > 0.2 || (double)__fdiv_rn((float)sOut, deg) > 0.2;
     for (uint64_t t = i; t < j; t++) {
-        const uint8_t a = G.adj[t];
+        const uint8_t a = adj_at(t);
         const int bi = a & 7, ai = (a >> 3) & 7;
         if ((nIn > 1 && bi != maxIn) || (nOut > 1 && ai != maxOut) || ratio) {
             // filteredReads; the refilter step turns it into BAD (the label is a
@@ -125,28 +121,69 @@ __device__ int filter_run(const GParams &G, uint64_t i)
     return 1;
 }
 
-// grid-stride over tuples; run statistics are summed per block, one atomic each
-__global__ __launch_bounds__(256) void filter_kernel(GParams G)
+// Filter over tiles of 2048 tuples staged in LDS (keys and adj read once):
+// lane l owns tuples 8l..8l+7 of the tile and evaluates the runs that start
+// there, reading past the tile end only for a run that crosses it.  Besides
+// the filter's effects it records two bitmaps for the union: every run start
+// (segment starts included) and the starts of the runs the union must join
+// (k-mer not zero and not zeroed, two tuples or more).
+constexpr int kFLanes = 256, kFPer = 8, kFTile = kFLanes * kFPer;
+
+__global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_t *__restrict__ rs_bits,
+                                                              uint64_t *__restrict__ st_bits)
 {
-    __shared__ uint32_t s_ev[256 / kWave], s_z[256 / kWave];
-    uint32_t ev = 0, z = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
-        const int r = filter_run(G, i);
-        ev += r >= 1;
-        z += r == 2;
+    __shared__ uint64_t s_key[kFTile];
+    __shared__ uint8_t s_adj[kFTile];
+    __shared__ unsigned long long s_rs[kFTile / 64], s_st[kFTile / 64];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
+    const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);
+    for (uint32_t j = threadIdx.x; j < cnt; j += kFLanes) {
+        s_key[j] = G.keys[t0 + j];
+        s_adj[j] = G.adj[t0 + j];
     }
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        ev += __shfl_down(ev, d, kWave);
-        z += __shfl_down(z, d, kWave);
+    if (threadIdx.x < kFTile / 64) {
+        s_rs[threadIdx.x] = 0;
+        s_st[threadIdx.x] = 0;
     }
-    if (__lane_id() == 0) { s_ev[threadIdx.x / kWave] = ev; s_z[threadIdx.x / kWave] = z; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t a = 0, b = 0;
-        for (int w = 0; w < 256 / kWave; w++) { a += s_ev[w]; b += s_z[w]; }
-        if (a) atomicAdd(&G.stats[0], (unsigned long long)a);
-        if (b) atomicAdd(&G.stats[1], (unsigned long long)b);
+    const uint64_t prevk = t0 ? G.keys[t0 - 1] : 0;
+    uint32_t ev = 0, z = 0;
+    uint64_t rs = 0, st = 0;
+    const uint32_t jb = threadIdx.x * kFPer;
+    uint64_t s0 = 0, s1 = 0;
+    for (int q = 0; q < kFPer; q++) {
+        const uint32_t j = jb + q;
+        if (j >= cnt) break;
+        const uint64_t i = t0 + j;
+        if (q == 0 || i >= s1) seg_of(G, i, s0, s1);
+        const uint64_t k = s_key[j];
+        if (i != s0 && (j ? s_key[j - 1] : prevk) == k) continue;  // not a run start
+        st |= 1ull << q;
+        uint32_t je = j + 1;
+        while (je < cnt && t0 + je < s1 && s_key[je] == k) je++;
+        uint64_t e = t0 + je;
+        if (je == cnt)
+            while (e < s1 && G.keys[e] == k) e++;
+        bool zeroed = false;
+        if (e != s1) {  // the last run of a segment is never evaluated
+            ev++;
+            const int r = eval_run(G, i, e, [&](uint64_t t) -> uint8_t {
+                return t - t0 < cnt ? s_adj[t - t0] : G.adj[t];
+            });
+            zeroed = r == 2;
+            z += zeroed;
+        }
+        if (k != 0 && !zeroed && e - i >= 2) rs |= 1ull << q;
+    }
+    if (st) atomicOr(&s_st[jb >> 6], (unsigned long long)(st << (jb & 63)));
+    if (rs) atomicOr(&s_rs[jb >> 6], (unsigned long long)(rs << (jb & 63)));
+    block_add(ev, &G.stats[0]);  // synchronises the workgroup
+    block_add(z, &G.stats[1]);
+    __syncthreads();
+    const uint64_t w0 = t0 / 64;
+    if (threadIdx.x < (cnt + 63) / 64) {
+        rs_bits[w0 + threadIdx.x] = s_rs[threadIdx.x];
+        st_bits[w0 + threadIdx.x] = s_st[threadIdx.x];
     }
 }
 
@@ -215,23 +252,31 @@ __global__ void bad_bits_kernel(const uint8_t *__restrict__ bad, uint64_t N, uin
 
 __device__ __forceinline__ bool is_bad(const uint64_t *bits, uint32_t x) { return (bits[x >> 6] >> (x & 63)) & 1; }
 
-__global__ void union_kernel(GParams G, const uint64_t *__restrict__ badbits)
+// One lane per tuple; a lane whose tuple starts a run to join finds the run's
+// end in the run-start bitmap and star-joins the non-BAD labels to the first
+// non-BAD one (metapart.c:2539-2616).
+__global__ void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits, const uint64_t *__restrict__ st_bits,
+                             const uint64_t *__restrict__ badbits)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
-        uint64_t s0, s1;
-        seg_of(G, i, s0, s1);
-        const uint64_t k = G.keys[i];
-        if (i != s0 && G.keys[i - 1] == k) continue;
-        if (k == 0) continue;  // zeroed run or the all-A k-mer (metapart.c:2574)
-        if (i + 1 >= s1 || G.keys[i + 1] != k) continue;  // one tuple: nothing to join
-        uint64_t j = i + 1;
-        while (j < s1 && G.keys[j] == k) j++;
+        if (!((rs_bits[i >> 6] >> (i & 63)) & 1)) continue;
+        uint64_t q = i >> 6;
+        uint64_t m = st_bits[q] & ~((2ull << (i & 63)) - 1);  // run starts after i in its word
+        uint64_t e = G.n;
+        for (;;) {
+            if (m) {
+                e = min(G.n, (q << 6) + (uint64_t)__builtin_ctzll(m));
+                break;
+            }
+            if (++q << 6 >= G.n) break;
+            m = st_bits[q];
+        }
         uint64_t c = i;
-        while (c < j && is_bad(badbits, G.lab[c])) c++;
-        if (c >= j) continue;
+        while (c < e && is_bad(badbits, G.lab[c])) c++;
+        if (c >= e) continue;
         const uint32_t a = G.lab[c];
-        for (uint64_t t = c + 1; t < j; t++) {
+        for (uint64_t t = c + 1; t < e; t++) {
             const uint32_t b = G.lab[t];
             if (b != a && !is_bad(badbits, b)) uf_union(G.parent, a, b);
         }
@@ -510,9 +555,12 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
     G.parent = (uint32_t *)ctx->parent.p;
     G.stats = d_stats;
     hipEventRecord(ctx->ev[2], ctx->stream);
+    const uint64_t rwords = (n + 63) / 64 + 1;
+    MP_TRY(ensure(ctx, ctx->runbits, 2 * rwords * sizeof(uint64_t)));
+    uint64_t *rs_bits = (uint64_t *)ctx->runbits.p, *st_bits = rs_bits + rwords;
     if (n) {
-        const uint64_t fb = std::min<uint64_t>(nblk(n), 256 * 16);
-        hipLaunchKernelGGL(filter_kernel, dim3((unsigned)fb), dim3(256), 0, ctx->stream, G);
+        hipLaunchKernelGGL(filter_tile_kernel, dim3((unsigned)((n + kFTile - 1) / kFTile)), dim3(kFLanes), 0,
+                           ctx->stream, G, rs_bits, st_bits);
         MP_HIP(hipGetLastError());
         hipLaunchKernelGGL(apply_zero_kernel, dim3(nblk(nwords)), dim3(256), 0, ctx->stream, keys,
                            (const uint32_t *)ctx->zbits.p, nwords);
@@ -555,7 +603,8 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
                            badbits);
         MP_HIP(hipGetLastError());
         const uint64_t ub = std::min<uint64_t>(nblk(n), 256 * 16);
-        hipLaunchKernelGGL(union_kernel, dim3((unsigned)ub), dim3(256), 0, ctx->stream, G, (const uint64_t *)badbits);
+        hipLaunchKernelGGL(union_kernel, dim3((unsigned)ub), dim3(256), 0, ctx->stream, G, (const uint64_t *)rs_bits,
+                           (const uint64_t *)st_bits, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.parent, (const uint8_t *)G.bad, N);

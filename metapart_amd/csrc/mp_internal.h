@@ -35,10 +35,12 @@ __device__ __forceinline__ uint64_t lanemask_lt()
 }
 
 // Adds v summed over the workgroup to *ctr with one atomic (blockDim.x a
-// multiple of 64, at most 1024; every lane of the workgroup must call it).
+// multiple of 64, at most 1024; every lane of the workgroup must call it; it
+// synchronises the workgroup, so calls may follow each other).
 __device__ __forceinline__ void block_add(uint32_t v, unsigned long long *ctr)
 {
     __shared__ uint32_t s_part[16];
+    __syncthreads();  // a previous call may still be reading s_part
     for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_down(v, d, kWave);
     if (__lane_id() == 0) s_part[threadIdx.x / kWave] = v;
     __syncthreads();
@@ -99,11 +101,16 @@ struct mp_ctx {
     mp::DevArray gparent, gsize;  // pair step: components of the candidate-pair graph
     // scan buffers (ping-pong)
     mp::DevArray keys[2], lab[2], adj[2], zbits;
+    mp::DevArray runbits;  // filter -> union: starts of runs to join, starts of all runs
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
     // writer
     mp::DevArray out, items[2], ilens[2], iaux[2];
+    // pinned host staging of large uploads (two buffers, one copy in flight)
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_used[2] = {false, false};
     std::vector<uint64_t> chunk_ofs;  // byte offset of each chunk (C+1)
     int cur = 0;               // buffer holding the last scan's sorted tuples
     uint64_t last_n = 0;       // tuples of the last scan
