@@ -54,7 +54,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->scanst, &ctx->hpart, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -312,18 +312,38 @@ int mp_dump_tuples(mp_ctx *ctx, uint8_t *dst, uint64_t cap, uint64_t *n)
     MP_HIP(hipSetDevice(ctx->device));
     const uint64_t m = ctx->last_n;
     std::vector<uint64_t> k(m);
-    std::vector<uint32_t> l(m);
+    std::vector<uint32_t> l(m), z((m + 31) / 32 + 1);
     std::vector<uint8_t> a(m);
     MP_HIP(hipMemcpyAsync(k.data(), ctx->keys[ctx->cur].p, m * 8, hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipMemcpyAsync(l.data(), ctx->lab[ctx->cur].p, m * 4, hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipMemcpyAsync(a.data(), ctx->adj[ctx->cur].p, m, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(z.data(), ctx->zbits.p, z.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
+    // the device sorts by k-mer only (runs in generation order): order each run
+    // by adj, stably, for the reference's (k-mer, adj) order; batches of
+    // samples also break runs at sample boundaries
+    std::vector<uint64_t> cuts;
+    if (ctx->batch) cuts = ctx->sample_tuples;
+    std::vector<uint64_t> idx(m);
+    for (uint64_t i = 0; i < m; i++) idx[i] = i;
+    size_t c = 0;
+    for (uint64_t i = 0; i < m;) {
+        while (c < cuts.size() && cuts[c] <= i) c++;
+        const uint64_t lim = c < cuts.size() ? cuts[c] : m;
+        uint64_t j = i + 1;
+        while (j < lim && k[j] == k[i]) j++;
+        if (j - i > 1)
+            std::stable_sort(idx.begin() + i, idx.begin() + j, [&](uint64_t x, uint64_t y) { return a[x] < a[y]; });
+        i = j;
+    }
     for (uint64_t i = 0; i < m; i++) {  // enumerateKmers.c:352-365 layout
+        const uint64_t s = idx[i];
+        const uint64_t key = ((z[s >> 5] >> (s & 31)) & 1) ? 0 : k[s];  // zeroed by the filter
         uint8_t *t = dst + 14 * i;
-        memcpy(t, &k[i], 8);
-        t[8] = a[i];
+        memcpy(t, &key, 8);
+        t[8] = a[s];
         t[9] = 0;  // id / UINT32_MAX: ids are 32-bit here
-        memcpy(t + 10, &l[i], 4);
+        memcpy(t + 10, &l[s], 4);
     }
     return 0;
 }
@@ -395,7 +415,7 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
                           ctx->stream));
     std::vector<uint64_t> tpos(ns + 1, 0);
     int passes = 0;
-    const int rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x3FF, &passes, d_first, ns,
+    const int rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x3FE, &passes, d_first, ns,
                                       tpos.data());
     MP_HIP(hipFreeAsync(d_first, ctx->stream));
     MP_TRY(rc);

@@ -8,9 +8,8 @@
 //
 // Filter (markMischiefKmers, metapart.c:2270-2519): tiles of tuples staged in
 // LDS; the lane holding a run's first tuple evaluates the run unless it is
-// the last run of its segment.  Zeroing is recorded in a bitmap and applied
-// afterwards, so no lane sees a zeroed key while another lane is still
-// finding run starts.
+// the last run of its segment.  Zeroing is recorded in a bitmap (the keys stay
+// as sorted; the union and the quirk read the bitmap, the tuple dump applies it).
 //
 // Union (unionFindPar2 + unionFind, metapart.c:2520-2800): the reference
 // star-unions the non-BAD labels of each run and always hooks the smaller root
@@ -187,17 +186,6 @@ __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_
     }
 }
 
-__global__ void apply_zero_kernel(uint64_t *__restrict__ keys, const uint32_t *__restrict__ zbits, uint64_t nwords)
-{
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwords) return;
-    uint32_t m = zbits[w];
-    while (m) {
-        const int b = __ffs(m) - 1;
-        keys[(w << 5) + b] = 0;
-        m &= m - 1;
-    }
-}
 
 // metapart.c:2500-2517: a read filtered in any scan is BAD again after the filter
 __global__ void refilter_kernel(uint8_t *__restrict__ bad, const uint8_t *__restrict__ filtered,
@@ -283,23 +271,59 @@ __global__ void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits, co
     }
 }
 
-// First surviving tuple of each segment (non-BAD label, non-zero key).
+__device__ __forceinline__ bool zeroed_at(const GParams &G, uint64_t t) { return (G.zbits[t >> 5] >> (t & 31)) & 1; }
+
+// The tuples are sorted by k-mer alone (stable, so each run is in generation
+// order); the reference's order also sorts a run by adj.  Where that order
+// matters, the first tuple of a run in the reference's order is the one with
+// the smallest adj, the earliest on ties.
+
+// First surviving tuple of each segment in the reference's order: within the
+// first run holding a survivor (label not BAD, k-mer neither 0 nor zeroed),
+// the survivor with the smallest adj.
 __global__ void first_alive_kernel(GParams G, uint64_t *__restrict__ first)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= G.T) return;
     uint64_t i = G.seg[t];
     const uint64_t e = G.seg[t + 1];
-    while (i < e && (G.bad[G.lab[i]] || G.keys[i] == 0)) i++;
-    first[t] = i < e ? i : ~0ull;
+    first[t] = ~0ull;
+    while (i < e) {
+        const uint64_t k = G.keys[i];
+        const bool dead = k == 0 || zeroed_at(G, i);
+        uint64_t j = i, best = ~0ull;
+        uint32_t ba = 256;
+        for (; j < e && G.keys[j] == k; j++)
+            if (!dead && G.adj[j] < ba && !G.bad[G.lab[j]]) {
+                ba = G.adj[j];
+                best = j;
+            }
+        if (best != ~0ull) {
+            first[t] = best;
+            return;
+        }
+        i = j;
+    }
 }
 
 __global__ void quirk_union_kernel(uint32_t *parent, uint32_t a, uint32_t b) { uf_union(parent, a, b); }
 
+// Labels to roots in two kernels.  halve_kernel walks every read to its root
+// with path halving (deep components get short paths, the walks share work);
+// compress_kernel then walks the short paths without writing on the way and
+// stores each read's root.  Halving writes in the same kernel as the final
+// stores could put a stale ancestor back over a read's stored root.
+__global__ void halve_kernel(uint32_t *parent, const uint8_t *__restrict__ bad, uint64_t N)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || (bad && bad[i])) return;
+    (void)uf_find(parent, (uint32_t)i);
+}
+
 __global__ void compress_kernel(uint32_t *parent, const uint8_t *__restrict__ bad, uint64_t N)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || bad[i]) return;
+    if (i >= N || (bad && bad[i])) return;
     uint32_t x = (uint32_t)i;
     for (;;) {
         const uint32_t p = parent[x];
@@ -381,19 +405,6 @@ __global__ void pair_graph_kernel(const uint32_t *__restrict__ parent, const uin
     if (j < P && pair_candidate(parent, sizes, j, a, b)) uf_union(gparent, a, b);
 }
 
-__global__ void compress_all_kernel(uint32_t *parent, uint64_t N)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    uint32_t x = (uint32_t)i;
-    for (;;) {
-        const uint32_t p = ld_agent(&parent[x]);
-        if (p == x) break;
-        x = p;
-    }
-    __hip_atomic_store(&parent[i], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // 1 for a candidate pair inside a candidate-graph component of > 1000 reads
 __global__ void pair_classify_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
                                      const uint32_t *__restrict__ gparent, const uint32_t *__restrict__ gsize,
@@ -405,11 +416,35 @@ __global__ void pair_classify_kernel(const uint32_t *__restrict__ parent, const 
     ordered[j] = pair_candidate(parent, sizes, j, a, b) && gsize[gparent[a]] > MP_PAIR_GATE;
 }
 
+// The ordered pass works on compact ids: the roots of the candidate-graph
+// components of > 1000 reads, numbered in id order, so "the larger id becomes
+// the root" is the same comparison on compact ids.
+__global__ void big_root_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ gparent,
+                                const uint32_t *__restrict__ gsize, uint32_t *__restrict__ flag, uint64_t N)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < N) flag[x] = parent[x] == (uint32_t)x && gsize[gparent[x]] > MP_PAIR_GATE;
+}
+
+// ordered pairs, in pair order, as (compact id of read 2j's root, of read 2j+1's root)
 __global__ void pair_list_kernel(const uint32_t *__restrict__ ordered, const uint64_t *__restrict__ off, uint64_t P,
-                                 uint64_t *__restrict__ idx)
+                                 const uint32_t *__restrict__ parent, const uint64_t *__restrict__ cid,
+                                 uint64_t *__restrict__ pairs)
 {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < P && ordered[j]) idx[off[j]] = j;
+    if (j < P && ordered[j])
+        pairs[off[j]] = (uint64_t)(uint32_t)cid[parent[2 * j]] | ((uint64_t)(uint32_t)cid[parent[2 * j + 1]] << 32);
+}
+
+__global__ void compact_roots_kernel(const uint32_t *__restrict__ flag, const uint64_t *__restrict__ cid,
+                                     const uint32_t *__restrict__ sizes, uint32_t *__restrict__ csize,
+                                     uint32_t *__restrict__ corig, uint64_t N)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < N && flag[x]) {
+        csize[cid[x]] = sizes[x];
+        corig[cid[x]] = (uint32_t)x;
+    }
 }
 
 // Reads of candidate-graph components of <= 1000 reads take the component's
@@ -429,12 +464,14 @@ __global__ void pair_apply_kernel(uint32_t *__restrict__ parent, const uint32_t 
     block_add(cnt, unions);
 }
 
-// Reads of the larger components take the labels of the ordered host pass.
-__global__ void pair_merge_kernel(uint32_t *__restrict__ parent, const uint32_t *__restrict__ host_parent,
-                                  const uint32_t *__restrict__ gparent, const uint32_t *__restrict__ gsize, uint64_t N)
+// Reads of the larger candidate-graph components take the roots the ordered
+// host pass gave their initial roots (cfinal: compact id -> final compact root).
+__global__ void pair_merge_kernel(uint32_t *__restrict__ parent, const uint32_t *__restrict__ gparent,
+                                  const uint32_t *__restrict__ gsize, const uint64_t *__restrict__ cid,
+                                  const uint32_t *__restrict__ cfinal, const uint32_t *__restrict__ corig, uint64_t N)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N && gsize[gparent[i]] > MP_PAIR_GATE) parent[i] = host_parent[i];
+    if (i < N && gsize[gparent[i]] > MP_PAIR_GATE) parent[i] = corig[cfinal[cid[parent[i]]]];
 }
 
 // Per-sample partition plans of a batch: W[p] over the sample's own read ids.
@@ -502,8 +539,19 @@ __global__ __launch_bounds__(256) void sample_stats_kernel(const uint32_t *__res
 
 static inline unsigned nblk(uint64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-// Per segment: first surviving tuple, its k-mer and label, and the last k-mer,
-// first label and that label's BAD flag of the segment's quirk group.
+// every read (except BAD ones, when bad is given) to its root
+static int compress_labels(mp_ctx *ctx, uint32_t *parent, const uint8_t *bad, uint64_t N)
+{
+    if (N == 0) return 0;
+    hipLaunchKernelGGL(halve_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, bad, N);
+    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, bad, N);
+    MP_HIP(hipGetLastError());
+    return 0;
+}
+
+// Per segment: first surviving tuple, its k-mer and label, and the last k-mer
+// (0 if zeroed), first label and that label's BAD flag of the segment's quirk
+// group, all in the reference's tuple order.
 __global__ void quirk_info_kernel(GParams G, const uint64_t *__restrict__ first, const uint64_t *__restrict__ grp,
                                   uint64_t *__restrict__ out)
 {
@@ -514,8 +562,17 @@ __global__ void quirk_info_kernel(GParams G, const uint64_t *__restrict__ first,
     o[0] = f;
     o[1] = f != ~0ull ? G.keys[f] : 0;
     o[2] = f != ~0ull ? G.lab[f] : 0;
-    o[3] = ge > gs ? G.keys[ge - 1] : 0;
-    const uint32_t L0 = ge > gs ? G.lab[gs] : 0;
+    o[3] = ge > gs && !zeroed_at(G, ge - 1) ? G.keys[ge - 1] : 0;
+    uint32_t L0 = 0;
+    if (ge > gs) {  // tuple 0 of the group: smallest adj of its first run
+        const uint64_t k = G.keys[gs];
+        uint32_t ba = 256;
+        for (uint64_t j = gs; j < ge && G.keys[j] == k; j++)
+            if (G.adj[j] < ba) {
+                ba = G.adj[j];
+                L0 = G.lab[j];
+            }
+    }
     o[4] = (uint64_t)L0 | ((uint64_t)(ge > gs && G.bad[L0]) << 32) | ((uint64_t)(ge > gs) << 33);
 }
 
@@ -562,9 +619,6 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
         hipLaunchKernelGGL(filter_tile_kernel, dim3((unsigned)((n + kFTile - 1) / kFTile)), dim3(kFLanes), 0,
                            ctx->stream, G, rs_bits, st_bits);
         MP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(apply_zero_kernel, dim3(nblk(nwords)), dim3(256), 0, ctx->stream, keys,
-                           (const uint32_t *)ctx->zbits.p, nwords);
-        MP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(refilter_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, G.bad,
                        (const uint8_t *)G.filtered, G.parent, N, d_stats + 2);
@@ -607,7 +661,7 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
                            (const uint64_t *)st_bits, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, G.parent, (const uint8_t *)G.bad, N);
+    MP_TRY(compress_labels(ctx, G.parent, (const uint8_t *)G.bad, N));
     MP_HIP(hipGetLastError());
     hipEventRecord(ctx->ev[4], ctx->stream);
     unsigned long long hs[4] = {0, 0, 0, 0};
@@ -686,7 +740,7 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     uint32_t *sizes = (uint32_t *)ctx->sizes.p;
     hipEventRecord(ctx->ev[5], ctx->stream);
     // sizes after the last scan (metapart.c:4010-4021)
-    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, (const uint8_t *)ctx->bad.p, N);
+    MP_TRY(compress_labels(ctx, parent, (const uint8_t *)ctx->bad.p, N));
     MP_TRY(fill_u32(ctx, sizes, 0, N));
     hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent, sizes, N);
     MP_HIP(hipGetLastError());
@@ -712,13 +766,16 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     uint32_t *gparent = (uint32_t *)ctx->gparent.p, *gsize = (uint32_t *)ctx->gsize.p;
     uint32_t *ordered = (uint32_t *)ctx->wg_cnt.p;
     uint64_t n_ordered = 0;
-    std::vector<uint32_t> hp, hs;
-    std::vector<uint64_t> hidx;
+    std::vector<uint64_t> hpairs;
+    std::vector<uint32_t> cs;
+    uint64_t M = 0;  // roots in candidate-graph components of > 1000 reads
+    uint32_t *d_flag = nullptr, *d_csize = nullptr, *d_corig = nullptr, *d_cfinal = nullptr;
+    uint64_t *d_cid = nullptr, *d_pairs = nullptr;
     if (P) {
         MP_HIP(hipMemcpyAsync(gparent, parent, N * 4, hipMemcpyDeviceToDevice, ctx->stream));
         hipLaunchKernelGGL(pair_graph_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
                            (const uint32_t *)sizes, gparent, P);
-        hipLaunchKernelGGL(compress_all_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, gparent, N);
+        MP_TRY(compress_labels(ctx, gparent, nullptr, N));
         MP_TRY(fill_u32(ctx, gsize, 0, N));
         hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)gparent, gsize,
                            N);
@@ -729,18 +786,32 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         MP_HIP(hipMemcpyAsync(&n_ordered, (uint64_t *)ctx->wg_off.p + P, sizeof(uint64_t), hipMemcpyDeviceToHost,
                               ctx->stream));
         MP_HIP(hipStreamSynchronize(ctx->stream));
-        if (n_ordered) {  // the ordered pairs, and the labels and sizes they start from
-            MP_TRY(ensure(ctx, ctx->scratch, n_ordered * sizeof(uint64_t) + 64));
-            uint64_t *d_idx = (uint64_t *)ctx->scratch.p;
-            hipLaunchKernelGGL(pair_list_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)ordered,
-                               (const uint64_t *)ctx->wg_off.p, P, d_idx);
+        if (n_ordered) {  // the ordered pairs on compact root ids, and those roots' sizes
+            MP_TRY(ensure(ctx, ctx->pairc, N * 4 + (N + 1) * 8 + n_ordered * 8 + 3 * N * 4 + 256));
+            uint8_t *pc = (uint8_t *)ctx->pairc.p;
+            d_flag = (uint32_t *)pc;
+            d_cid = (uint64_t *)(pc + ((N * 4 + 7) & ~7ull));
+            d_pairs = d_cid + (N + 1);
+            d_csize = (uint32_t *)(d_pairs + n_ordered);
+            d_corig = d_csize + N;
+            d_cfinal = d_corig + N;
+            hipLaunchKernelGGL(big_root_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+                               (const uint32_t *)gparent, (const uint32_t *)gsize, d_flag, N);
             MP_HIP(hipGetLastError());
-            hp.resize(N);
-            hs.resize(N);
-            hidx.resize(n_ordered);
-            MP_HIP(hipMemcpyAsync(hidx.data(), d_idx, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(hp.data(), parent, N * 4, hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(hs.data(), sizes, N * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_TRY(scan_u32(ctx, d_flag, d_cid, N));
+            hipLaunchKernelGGL(pair_list_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)ordered,
+                               (const uint64_t *)ctx->wg_off.p, P, (const uint32_t *)parent, (const uint64_t *)d_cid,
+                               d_pairs);
+            hipLaunchKernelGGL(compact_roots_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream,
+                               (const uint32_t *)d_flag, (const uint64_t *)d_cid, (const uint32_t *)sizes, d_csize,
+                               d_corig, N);
+            MP_HIP(hipGetLastError());
+            MP_HIP(hipMemcpyAsync(&M, d_cid + N, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipStreamSynchronize(ctx->stream));
+            hpairs.resize(n_ordered);
+            cs.resize(M);
+            MP_HIP(hipMemcpyAsync(hpairs.data(), d_pairs, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(cs.data(), d_csize, M * 4, hipMemcpyDeviceToHost, ctx->stream));
         }
         hipLaunchKernelGGL(pair_apply_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, parent,
                            (const uint32_t *)gparent, (const uint32_t *)gsize, N, d_unions);
@@ -751,26 +822,29 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     if (n_ordered) {
         MP_HIP(hipStreamSynchronize(ctx->stream));
         const auto t0 = std::chrono::steady_clock::now();
+        std::vector<uint32_t> cp(M);
+        for (uint64_t c = 0; c < M; c++) cp[c] = (uint32_t)c;
         for (uint64_t q = 0; q < n_ordered; q++) {
-            const uint64_t i = 2 * hidx[q];
-            const uint32_t a = hfind(hp.data(), (uint32_t)i), b = hfind(hp.data(), (uint32_t)(i + 1));
+            const uint32_t a = hfind(cp.data(), (uint32_t)hpairs[q]), b = hfind(cp.data(), (uint32_t)(hpairs[q] >> 32));
             if (a == b) continue;
-            if (hs[a] <= MP_PAIR_GATE || hs[b] <= MP_PAIR_GATE) {
+            if (cs[a] <= MP_PAIR_GATE || cs[b] <= MP_PAIR_GATE) {
                 unions++;
-                if (a > b) { hp[b] = a; hs[a] += hs[b]; hs[b] = 0; }
-                else { hp[a] = b; hs[b] += hs[a]; hs[a] = 0; }
+                if (a > b) { cp[b] = a; cs[a] += cs[b]; cs[b] = 0; }
+                else { cp[a] = b; cs[b] += cs[a]; cs[a] = 0; }
             }
         }
+        for (uint64_t c = 0; c < M; c++) cp[c] = hfind(cp.data(), (uint32_t)c);
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        // sizes[] is recomputed below: it stages the host labels
-        MP_HIP(hipMemcpyAsync(sizes, hp.data(), N * 4, hipMemcpyHostToDevice, ctx->stream));
+        MP_HIP(hipMemcpyAsync(d_cfinal, cp.data(), M * 4, hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(pair_merge_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
-                           (const uint32_t *)sizes, (const uint32_t *)gparent, (const uint32_t *)gsize, N);
+                           (const uint32_t *)gparent, (const uint32_t *)gsize, (const uint64_t *)d_cid,
+                           (const uint32_t *)d_cfinal, (const uint32_t *)d_corig, N);
         MP_HIP(hipGetLastError());
+        MP_HIP(hipStreamSynchronize(ctx->stream));  // cp is read by the copy above
     }
     // every BAD read was reset to itself by the pair step
     MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
-    hipLaunchKernelGGL(compress_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent, (const uint8_t *)ctx->bad.p, N);
+    MP_TRY(compress_labels(ctx, parent, (const uint8_t *)ctx->bad.p, N));
     MP_TRY(fill_u32(ctx, sizes, 0, N));
     hipLaunchKernelGGL(sizes_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent, sizes, N);
     MP_HIP(hipGetLastError());

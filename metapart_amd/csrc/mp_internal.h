@@ -99,6 +99,7 @@ struct mp_ctx {
     // graph state
     mp::DevArray parent, bad, filtered, sizes;
     mp::DevArray gparent, gsize;  // pair step: components of the candidate-pair graph
+    mp::DevArray pairc;           // pair step: compact ids of the ordered pass
     // scan buffers (ping-pong)
     mp::DevArray keys[2], lab[2], adj[2], zbits;
     mp::DevArray runbits;  // filter -> union: starts of runs to join, starts of all runs

@@ -1,11 +1,15 @@
-// Stable LSD radix sort of the scan's tuples by (k-mer u64, adj u8).
+// Stable LSD radix sort of the scan's tuples by k-mer.
 //
 // The reference sorts 14-byte tuples with 9 stable byte passes per thread
 // segment: byte 8 (adj) first, then k-mer bytes 0..7 (sortKmerList and
 // radixPass, metapart.c:1494-1598, 2167-2198), after a range partition by
 // segment.  Segments are contiguous bucket ranges of the k-mer's top bits, so
 // one global stable sort gives the same array.  Stability keeps generation
-// order among equal (k-mer, adj), as the reference's counting passes do.
+// order among equal keys, as the reference's counting passes do.  Only the
+// k-mer passes run: inside a run of equal k-mers the filter and the union do
+// not depend on the order, and the few places that need the reference's adj
+// order (the first-run quirk, the tuple dump) derive it from the run, which
+// is in generation order.  The adj pass stays available (pass 0).
 //
 // MI355X design: structure-of-arrays tuples (keys u64, labels u32, adj u8;
 // 13 bytes), one histogram read for every pass up front, then one "onesweep"
@@ -374,7 +378,9 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
 
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
 {
-    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FF, passes_out, nullptr, 0,
+    // k-mer bytes only: within a run nothing downstream depends on the adj
+    // order except the quirk's first tuples and the tuple dump, which derive it
+    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FE, passes_out, nullptr, 0,
                               nullptr);
 }
 

@@ -84,3 +84,24 @@ def test_error_paths_return_codes():
         with pytest.raises(mp.MPError) as e:
             small.load(good, [0, len(good)])
         assert e.value.code == -2
+
+
+@pytest.mark.gpu
+def test_scan_labels_are_deterministic_and_compressed():
+    """Lock-free union and compress: repeated runs give the same labels, and
+    after a scan every non-BAD read's label is a root (compression is complete)."""
+    print("gpu determinism: 3-minimizer scan, 250k reads, 4 repeats", flush=True)
+    buf = mp.synth(1, n_reads=250_000, seed=3)
+    seen = set()
+    for rep in range(4):
+        with mp.Context(k=27, min3=True) as ctx:
+            ctx.load(buf, mp.index_buffer(buf, 8))
+            hist, _, _, _ = ctx.count()
+            scan, thread = mp.plan_ranges(hist, 1, 1)
+            ctx.scan(0, int(scan[0]), int(scan[1]), thread, int(hist[int(scan[1])]))
+            par = ctx.parent().astype(np.int64)
+            bad = ctx.bad().astype(bool)
+        ok = ~bad
+        assert np.all(par[par[ok]] == par[ok])
+        seen.add(hashlib.sha256(par.tobytes()).hexdigest())
+    assert len(seen) == 1
