@@ -71,11 +71,35 @@ __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, in
 template <typename AdjAt>
 __device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at)
 {
-    uint32_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint64_t t = i; t < j; t++) {
-        const uint8_t a = adj_at(t);
-        in[a & 7]++;
-        out[(a >> 3) & 7]++;
+    uint32_t in[8], out[8];
+    if (j - i < 65536) {  // tallies in 16-bit fields: classes 0-3 and 4-7 in two words per side
+        uint64_t il = 0, ih = 0, ol = 0, oh = 0;
+        for (uint64_t t = i; t < j; t++) {
+            const uint32_t a = adj_at(t), ci = a & 7, co = (a >> 3) & 7;
+            const uint64_t bi = 1ull << (16 * (ci & 3)), bo = 1ull << (16 * (co & 3));
+            il += ci < 4 ? bi : 0;
+            ih += ci < 4 ? 0 : bi;
+            ol += co < 4 ? bo : 0;
+            oh += co < 4 ? 0 : bo;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            in[c] = (uint32_t)(il >> (16 * c)) & 0xFFFF;
+            in[c + 4] = (uint32_t)(ih >> (16 * c)) & 0xFFFF;
+            out[c] = (uint32_t)(ol >> (16 * c)) & 0xFFFF;
+            out[c + 4] = (uint32_t)(oh >> (16 * c)) & 0xFFFF;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 8; c++) in[c] = out[c] = 0;
+        for (uint64_t t = i; t < j; t++) {
+            const uint32_t a = adj_at(t);
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                in[c] += (a & 7) == (uint32_t)c;
+                out[c] += ((a >> 3) & 7) == (uint32_t)c;
+            }
+        }
     }
     int nIn = 0, nOut = 0, maxIn = 0, maxOut = 0;
     uint32_t mIn = 0, mOut = 0, sIn = 0, sOut = 0;
@@ -120,12 +144,14 @@ __device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at)
     return 1;
 }
 
-// Filter over tiles of 2048 tuples staged in LDS (keys and adj read once):
-// lane l owns tuples 8l..8l+7 of the tile and evaluates the runs that start
-// there, reading past the tile end only for a run that crosses it.  Besides
-// the filter's effects it records two bitmaps for the union: every run start
-// (segment starts included) and the starts of the runs the union must join
-// (k-mer not zero and not zeroed, two tuples or more).
+// Filter over tiles of 2048 tuples staged in LDS (keys and adj read once).
+// Lane l of the workgroup owns tuples l, l+256, ... of the tile (consecutive
+// lanes read consecutive keys, no bank conflicts) and evaluates the runs that
+// start there, reading past the tile end only for a run that crosses it.
+// Besides the filter's effects it records two bitmaps for the union, one
+// wave ballot per 64 tuples: every run start (segment starts included) and
+// the starts of the runs the union must join (k-mer not zero and not zeroed,
+// two tuples or more).
 constexpr int kFLanes = 256, kFPer = 8, kFTile = kFLanes * kFPer;
 
 __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_t *__restrict__ rs_bits,
@@ -133,59 +159,53 @@ __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_
 {
     __shared__ uint64_t s_key[kFTile];
     __shared__ uint8_t s_adj[kFTile];
-    __shared__ unsigned long long s_rs[kFTile / 64], s_st[kFTile / 64];
     const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
     const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);
     for (uint32_t j = threadIdx.x; j < cnt; j += kFLanes) {
         s_key[j] = G.keys[t0 + j];
         s_adj[j] = G.adj[t0 + j];
     }
-    if (threadIdx.x < kFTile / 64) {
-        s_rs[threadIdx.x] = 0;
-        s_st[threadIdx.x] = 0;
-    }
     __syncthreads();
     const uint64_t prevk = t0 ? G.keys[t0 - 1] : 0;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     uint32_t ev = 0, z = 0;
-    uint64_t rs = 0, st = 0;
-    const uint32_t jb = threadIdx.x * kFPer;
-    uint64_t s0 = 0, s1 = 0;
+    uint64_t s0 = 1, s1 = 0;  // no segment yet
     for (int q = 0; q < kFPer; q++) {
-        const uint32_t j = jb + q;
-        if (j >= cnt) break;
-        const uint64_t i = t0 + j;
-        if (q == 0 || i >= s1) seg_of(G, i, s0, s1);
-        const uint64_t k = s_key[j];
-        if (i != s0 && (j ? s_key[j - 1] : prevk) == k) continue;  // not a run start
-        st |= 1ull << q;
-        uint32_t je = j + 1;
-        while (je < cnt && t0 + je < s1 && s_key[je] == k) je++;
-        uint64_t e = t0 + je;
-        if (je == cnt)
-            while (e < s1 && G.keys[e] == k) e++;
-        bool zeroed = false;
-        if (e != s1) {  // the last run of a segment is never evaluated
-            ev++;
-            const int r = eval_run(G, i, e, [&](uint64_t t) -> uint8_t {
-                return t - t0 < cnt ? s_adj[t - t0] : G.adj[t];
-            });
-            zeroed = r == 2;
-            z += zeroed;
+        const uint32_t j = q * kFLanes + threadIdx.x;
+        bool st = false, rs = false;
+        if (j < cnt) {
+            const uint64_t i = t0 + j;
+            if (i < s0 || i >= s1) seg_of(G, i, s0, s1);
+            const uint64_t k = s_key[j];
+            if (i == s0 || (j ? s_key[j - 1] : prevk) != k) {
+                st = true;
+                uint32_t je = j + 1;
+                while (je < cnt && t0 + je < s1 && s_key[je] == k) je++;
+                uint64_t e = t0 + je;
+                if (je == cnt)
+                    while (e < s1 && G.keys[e] == k) e++;
+                bool zeroed = false;
+                if (e != s1) {  // the last run of a segment is never evaluated
+                    ev++;
+                    zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t {
+                                 return t - t0 < cnt ? s_adj[t - t0] : G.adj[t];
+                             }) == 2;
+                    z += zeroed;
+                }
+                rs = k != 0 && !zeroed && e - i >= 2;
+            }
         }
-        if (k != 0 && !zeroed && e - i >= 2) rs |= 1ull << q;
+        const uint64_t mst = __ballot(st), mrs = __ballot(rs);
+        const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
+        if (lane == 0 && jw < cnt) {
+            const uint64_t w = (t0 + jw) / 64;
+            st_bits[w] = mst;
+            rs_bits[w] = mrs;
+        }
     }
-    if (st) atomicOr(&s_st[jb >> 6], (unsigned long long)(st << (jb & 63)));
-    if (rs) atomicOr(&s_rs[jb >> 6], (unsigned long long)(rs << (jb & 63)));
-    block_add(ev, &G.stats[0]);  // synchronises the workgroup
+    block_add(ev, &G.stats[0]);
     block_add(z, &G.stats[1]);
-    __syncthreads();
-    const uint64_t w0 = t0 / 64;
-    if (threadIdx.x < (cnt + 63) / 64) {
-        rs_bits[w0 + threadIdx.x] = s_rs[threadIdx.x];
-        st_bits[w0 + threadIdx.x] = s_st[threadIdx.x];
-    }
 }
-
 
 // metapart.c:2500-2517: a read filtered in any scan is BAD again after the filter
 __global__ void refilter_kernel(uint8_t *__restrict__ bad, const uint8_t *__restrict__ filtered,
