@@ -18,9 +18,10 @@
 //     earlier tiles are always resident or done); two tiles fit a CU,
 //   - each wave ranks its 384 tuples stably with 8 ballots per item
 //     (bit-sliced digit match) and per-wave digit counters in LDS,
-//   - per-digit tile counts are published and the digit's global prefix is
-//     found by decoupled look-back over earlier tiles (epoch-tagged status
-//     words, so the status array is never cleared between passes),
+//   - per-digit tile counts (from the per-wave counters of the ranking) are
+//     published and the digit's global prefix is found by decoupled look-back
+//     over earlier tiles (epoch-tagged status words, so the status array is
+//     never cleared between passes),
 //   - tuples are reordered by digit in LDS and written out, so consecutive
 //     lanes write consecutive addresses of each digit's run.
 // Passes whose digit is constant over the scan (e.g. the top byte when K<29)
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     __shared__ uint64_t s_buf[kTile];
     __shared__ uint8_t s_dig[kTile];
     __shared__ uint16_t s_cnt[kSW][kRadix];
-    __shared__ uint32_t s_dstart[kRadix];  // tile histogram first, then digit starts
+    __shared__ uint32_t s_dstart[kRadix];  // digit starts in the tile
     __shared__ uint64_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / kWave];
     __shared__ uint32_t s_tile;
@@ -124,7 +125,6 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     const int wave = tid / kWave;
     const unsigned lane = __lane_id();
     for (int j = tid; j < kSW * kRadix; j += kSB) (&s_cnt[0][0])[j] = 0;
-    if (tid < kRadix) s_dstart[tid] = 0;
     if (tid == 0) s_tile = atomicAdd(P.tile_ctr, 1u);
     __syncthreads();
     const uint64_t tile = s_tile;
@@ -150,18 +150,6 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
             d[i] = kRadix;  // invalid
         }
     }
-#pragma unroll
-    for (int i = 0; i < kSI; i++)
-        if (d[i] < (uint32_t)kRadix) atomicAdd(&s_dstart[d[i]], 1u);
-    __syncthreads();
-    // publish this tile's digit counts before ranking, so later tiles can look
-    // back past it while it is still busy
-    uint32_t total = 0;
-    if (tid < kRadix) {
-        total = s_dstart[tid];
-        uint64_t *st = P.status + tile * kRadix + tid;
-        st_agent(st, st_pack(P.epoch, tile == 0 ? kStInc : kStAgg, total));
-    }
     // stable rank within the wave: item-major, lane-minor order
     const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -186,7 +174,9 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // per digit: exclusive offsets over waves and the tile-local digit start
+    // per digit: exclusive offsets over waves, the tile's count (published for
+    // later tiles' look-back), and the tile-local digit start
+    uint32_t total = 0;
     if (tid < kRadix) {
         uint32_t run = 0;
 #pragma unroll
@@ -195,6 +185,8 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
             s_cnt[w][tid] = (uint16_t)run;
             run += c;
         }
+        total = run;
+        st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, tile == 0 ? kStInc : kStAgg, total));
         uint32_t inc = total;
         for (int o = 1; o < kWave; o <<= 1) {
             const uint32_t v = __shfl_up(inc, o, kWave);
@@ -207,7 +199,10 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     if (tid < kRadix) {
         uint32_t woff = 0;
         for (int w = 0; w < wave; w++) woff += s_wsum[w];
-        s_dstart[tid] += woff;
+        const uint32_t ds = s_dstart[tid] + woff;
+        s_dstart[tid] = ds;
+#pragma unroll
+        for (int w = 0; w < kSW; w++) s_cnt[w][tid] = (uint16_t)(s_cnt[w][tid] + ds);  // tile position of wave w's first
         // decoupled look-back for digit tid
         uint64_t prefix = 0;
         if (tile != 0) {
@@ -222,14 +217,14 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
             }
             st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, kStInc, prefix + total));
         }
-        s_gbase[tid] = P.digit_base[tid] + prefix;
+        s_gbase[tid] = P.digit_base[tid] + prefix - ds;  // output index of tile position 0 of this digit
     }
     __syncthreads();
     // tile-local position of every item, then two staged write-outs
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
         if (d[i] < (uint32_t)kRadix) {
-            rank[i] += s_dstart[d[i]] + s_cnt[wave][d[i]];
+            rank[i] += s_cnt[wave][d[i]];
             s_buf[rank[i]] = k[i];
             s_dig[rank[i]] = (uint8_t)d[i];
         }
@@ -237,8 +232,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     __syncthreads();
     const uint32_t nvalid = (uint32_t)(P.n - base < (uint64_t)kTile ? P.n - base : (uint64_t)kTile);
     for (uint32_t j = tid; j < nvalid; j += kSB) {
-        const uint32_t dd = s_dig[j];
-        P.keys_out[s_gbase[dd] + (j - s_dstart[dd])] = s_buf[j];
+        P.keys_out[s_gbase[s_dig[j]] + j] = s_buf[j];
     }
     __syncthreads();
     uint32_t *s_lab = (uint32_t *)s_buf;
@@ -252,8 +246,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     }
     __syncthreads();
     for (uint32_t j = tid; j < nvalid; j += kSB) {
-        const uint32_t dd = s_dig[j];
-        const uint64_t g = s_gbase[dd] + (j - s_dstart[dd]);
+        const uint64_t g = s_gbase[s_dig[j]] + j;
         P.lab_out[g] = s_lab[j];
         P.adj_out[g] = s_adj[j];
     }
