@@ -84,6 +84,8 @@ typedef struct {
     float ms_enumerate, ms_sort, ms_filter, ms_union; /* device time */
     float ms_sort_pass_avg;   /* mean duration of one radix scatter pass */
     float ms_sort_hist;       /* the digit-histogram read before the passes */
+    float ms_sort_local;      /* the local pass ranking tuples inside high-bit groups (0: all passes global) */
+    uint64_t sort_long_groups;/* groups too long for the local pass, sorted by their own global passes */
 } mp_scan_stats;
 
 typedef struct {

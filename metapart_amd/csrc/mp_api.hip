@@ -257,6 +257,8 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     st.sort_passes = passes;
     st.ms_sort_pass_avg = ctx->sort_pass_avg;
     st.ms_sort_hist = ctx->sort_hist_ms;
+    st.ms_sort_local = ctx->local_ms;
+    st.sort_long_groups = ctx->local_long;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     if (stats) *stats = st;
@@ -415,8 +417,13 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
                           ctx->stream));
     std::vector<uint64_t> tpos(ns + 1, 0);
     int passes = 0;
-    const int rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x3FE, &passes, d_first, ns,
-                                      tpos.data());
+    int rc = radix_sort_tuples(ctx, n, &passes);
+    if (!rc) {  // then the sample of the label as the most significant digit
+        int sp = 0;
+        rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 1u << 9, &sp, d_first, ns,
+                                tpos.data());
+        passes += sp;
+    }
     MP_HIP(hipFreeAsync(d_first, ctx->stream));
     MP_TRY(rc);
     if (n <= 1) {  // the sort returned early: one tuple or none, all in the sample that owns it
@@ -442,6 +449,8 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     st.sort_passes = passes;
     st.ms_sort_pass_avg = ctx->sort_pass_avg;
     st.ms_sort_hist = ctx->sort_hist_ms;
+    st.ms_sort_local = ctx->local_ms;
+    st.sort_long_groups = ctx->local_long;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     ctx->last_n = n;

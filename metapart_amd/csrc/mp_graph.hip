@@ -68,8 +68,8 @@ __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, in
 // Evaluates run [i, j) (not the last run of its segment): the in/out
 // neighbour classes of its tuples decide whether the k-mer is zeroed (both
 // sides branch) or which labels are filtered.  Returns 1, or 2 when zeroed.
-template <typename AdjAt>
-__device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at)
+template <typename AdjAt, typename LabAt>
+__device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at, LabAt lab_at)
 {
     uint32_t in[8], out[8];
     if (j - i < 65536) {  // tallies in 16-bit fields: classes 0-3 and 4-7 in two words per side
@@ -137,14 +137,13 @@ __device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at)
         if ((nIn > 1 && bi != maxIn) || (nOut > 1 && ai != maxOut) || ratio) {
             // filteredReads; the refilter step turns it into BAD (the label is a
             // root here, so that step's parent reset changes nothing for it)
-            const uint32_t L = G.lab[t];
-            if (!G.filtered[L]) G.filtered[L] = 1;
+            G.filtered[lab_at(t)] = 1;
         }
     }
     return 1;
 }
 
-// Filter over tiles of 2048 tuples staged in LDS (keys and adj read once).
+// Filter over tiles of 2048 tuples staged in LDS (keys, labels and adj read once).
 // Lane l of the workgroup owns tuples l, l+256, ... of the tile (consecutive
 // lanes read consecutive keys, no bank conflicts) and evaluates the runs that
 // start there, reading past the tile end only for a run that crosses it.
@@ -158,11 +157,13 @@ __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_
                                                               uint64_t *__restrict__ st_bits)
 {
     __shared__ uint64_t s_key[kFTile];
+    __shared__ uint32_t s_lab[kFTile];
     __shared__ uint8_t s_adj[kFTile];
     const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
     const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);
     for (uint32_t j = threadIdx.x; j < cnt; j += kFLanes) {
         s_key[j] = G.keys[t0 + j];
+        s_lab[j] = G.lab[t0 + j];
         s_adj[j] = G.adj[t0 + j];
     }
     __syncthreads();
@@ -187,9 +188,10 @@ __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_
                 bool zeroed = false;
                 if (e != s1) {  // the last run of a segment is never evaluated
                     ev++;
-                    zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t {
-                                 return t - t0 < cnt ? s_adj[t - t0] : G.adj[t];
-                             }) == 2;
+                    zeroed = eval_run(
+                                 G, i, e,
+                                 [&](uint64_t t) -> uint32_t { return t - t0 < cnt ? s_adj[t - t0] : G.adj[t]; },
+                                 [&](uint64_t t) -> uint32_t { return t - t0 < cnt ? s_lab[t - t0] : G.lab[t]; }) == 2;
                     z += zeroed;
                 }
                 rs = k != 0 && !zeroed && e - i >= 2;

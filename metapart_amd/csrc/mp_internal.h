@@ -82,6 +82,8 @@ struct mp_ctx {
     hipEvent_t ev[8]{};
     hipEvent_t pev[24]{};  // per radix pass (timing of the dominant kernel)
     float sort_pass_avg = 0, sort_hist_ms = 0;
+    float local_ms = 0;        // local rank pass of the last sort
+    uint64_t local_long = 0;   // groups it handed to the long-group kernel
     std::string err;
     uint64_t budget = 0, held = 0;
     uint16_t epoch = 0;  // look-back status epoch of the radix sort

@@ -140,55 +140,47 @@ __global__ void rec_finalize_kernel(const uint8_t *__restrict__ fq, uint64_t n_r
     npos[r] = L > (uint64_t)K ? (uint32_t)(L - K + 1) : 0u;
 }
 
-// 64 bases per thread: two words of the 2-bit stream and one word of the N stream.
-__global__ __launch_bounds__(256) void pack_kernel(const uint8_t *__restrict__ fq, const uint64_t *__restrict__ seq_off,
-                                                   const uint64_t *__restrict__ base_off, uint64_t n_rec,
-                                                   uint64_t total_bases, uint64_t n_threads,
-                                                   uint64_t *__restrict__ packed, uint64_t *__restrict__ nmask)
+// Packing in two coalesced steps: every read's sequence bytes are copied to a
+// dense base array (one wave per read, consecutive lanes on consecutive
+// bytes), then each lane turns 64 dense bases (four aligned 16-byte loads)
+// into two words of the 2-bit stream and one word of the N stream.
+__global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restrict__ fq,
+                                                          const uint64_t *__restrict__ seq_off,
+                                                          const uint64_t *__restrict__ base_off, uint64_t n_rec,
+                                                          uint8_t *__restrict__ dense)
 {
-    // reads overlapping this workgroup's 256 x 64 bases: one search over all
-    // reads by the first lanes, then every lane searches only that range
-    __shared__ uint64_t s_r[2];
-    const uint64_t wb = (uint64_t)blockIdx.x * blockDim.x * 64;
-    if (threadIdx.x < 2) {
-        const uint64_t target = threadIdx.x == 0 ? wb : min(wb + (uint64_t)blockDim.x * 64, total_bases) - 1;
-        uint64_t lo = 0, hi = n_rec;
-        if (wb < total_bases) {
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi + 1) >> 1;
-                if (base_off[mid] <= target) lo = mid; else hi = mid - 1;
-            }
-        }
-        s_r[threadIdx.x] = lo;
-    }
-    __syncthreads();
+    const uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (r >= n_rec) return;
+    const uint64_t src = seq_off[r], dst = base_off[r], len = base_off[r + 1] - dst;
+    for (uint64_t b = __lane_id(); b < len; b += kWave) dense[dst + b] = fq[src + b];
+}
+
+__global__ __launch_bounds__(256) void pack_dense_kernel(const uint8_t *__restrict__ dense, uint64_t total_bases,
+                                                         uint64_t n_words, uint64_t *__restrict__ packed,
+                                                         uint64_t *__restrict__ nmask)
+{
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_threads) return;
-    const uint64_t b0 = t * 64;
+    if (t >= n_words) return;
     uint64_t w0 = 0, w1 = 0, nm = 0;
+    const uint64_t b0 = t * 64;
     if (b0 < total_bases) {
-        // read containing base b0: last r with base_off[r] <= b0
-        uint64_t lo = s_r[0], hi = s_r[1];
-        while (lo < hi) {
-            uint64_t mid = (lo + hi + 1) >> 1;
-            if (base_off[mid] <= b0) lo = mid; else hi = mid - 1;
-        }
-        uint64_t r = lo;
-        uint64_t rb = base_off[r], re = base_off[r + 1], so = seq_off[r];
-        for (int j = 0; j < 64; j++) {
-            const uint64_t b = b0 + (uint64_t)j;
-            if (b >= total_bases) break;
-            while (b >= re) {
-                r++;
-                rb = re;
-                re = base_off[r + 1];
-                so = seq_off[r];
+        const uint4 *v = (const uint4 *)(dense + b0);
+        const uint32_t nb = (uint32_t)min((uint64_t)64, total_bases - b0);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = v[q];
+            const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int j = q * 16 + e;
+                const uint8_t c = (uint8_t)(wd[e >> 2] >> (8 * (e & 3)));
+                if ((uint32_t)j < nb) {
+                    const uint64_t code = (uint64_t)((0x2310u >> (((c & 6) >> 1) * 4)) & 3);  // {0,1,3,2}[(c&6)>>1]
+                    if (j < 32) w0 |= code << (62 - 2 * j);
+                    else w1 |= code << (62 - 2 * (j - 32));
+                    if (c == 'N') nm |= 1ull << (63 - j);
+                }
             }
-            const uint8_t c = fq[so + (b - rb)];
-            const uint64_t code = (uint64_t)((0x2310u >> (((c & 6) >> 1) * 4)) & 3);  // {0,1,3,2}[(c&6)>>1]
-            if (j < 32) w0 |= code << (62 - 2 * j);
-            else w1 |= code << (62 - 2 * (j - 32));
-            if (c == 'N') nm |= 1ull << (63 - j);
         }
     }
     packed[2 * t] = w0;
@@ -312,9 +304,16 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     const uint64_t nthreads = (tb + 63) / 64 + 2;  // two zero words of padding after the stream
     MP_TRY(ensure(ctx, ctx->packed, nthreads * 2 * sizeof(uint64_t)));
     MP_TRY(ensure(ctx, ctx->nmask, nthreads * sizeof(uint64_t)));
-    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, ctx->stream, fq,
-                       (const uint64_t *)ctx->seq_off.p, (const uint64_t *)ctx->base_off.p, N, tb, nthreads,
-                       (uint64_t *)ctx->packed.p, (uint64_t *)ctx->nmask.p);
+    // the dense bases stage in the second key buffer, unused until the first sort
+    MP_TRY(ensure(ctx, ctx->keys[1], nthreads * 64 + 64));
+    uint8_t *dense = (uint8_t *)ctx->keys[1].p;
+    if (N) {
+        hipLaunchKernelGGL(dense_bases_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream, fq,
+                           (const uint64_t *)ctx->seq_off.p, (const uint64_t *)ctx->base_off.p, N, dense);
+        MP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(pack_dense_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const uint8_t *)dense, tb, nthreads, (uint64_t *)ctx->packed.p, (uint64_t *)ctx->nmask.p);
     MP_HIP(hipGetLastError());
     MP_HIP(hipStreamSynchronize(ctx->stream));
     return 0;

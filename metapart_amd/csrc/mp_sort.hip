@@ -32,6 +32,7 @@
 // every sample's tuples in (k-mer, adj) order, samples one after another, the
 // same as sorting each sample alone.
 #include <cstdlib>
+#include <vector>
 
 #include "mp_internal.h"
 
@@ -369,12 +370,206 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     return 0;
 }
 
+// ---------------------------------------------------------------- hybrid order
+// Global passes on the k-mer's high bytes only, then one local pass: after
+// the high passes (stable, from generation order) tuples with equal high bits
+// -- a "group" -- are contiguous and in generation order, and with about 30
+// high bits a group holds a few tuples.  The local pass gives each tuple its
+// rank inside its group by (k-mer, position) and writes it there: the same
+// array the low-byte passes would have produced, for the cost of one pass.
+// A workgroup owns the groups that start in its 2048-tuple window (staged in
+// LDS with up to 2048 more for the last group); a longer group goes to a list
+// and is sorted by its low bytes with the global passes on its own range.
+constexpr int kLWin = 2048, kLMax = 2048, kLCap = kLWin + kLMax, kLThreads = 256;
+
+struct LocalPass {
+    const uint64_t *keys_in;
+    const uint32_t *lab_in;
+    const uint8_t *adj_in;
+    uint64_t *keys_out;
+    uint32_t *lab_out;
+    uint8_t *adj_out;
+    uint64_t n;
+    int shift;                  // high bits: key >> shift
+    uint64_t *ovf;              // starts of groups longer than kLMax
+    unsigned long long *novf;
+};
+
+__device__ __forceinline__ bool group_start(const uint64_t *keys, uint64_t i, int shift)
+{
+    return i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
+}
+
+__global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
+{
+    __shared__ uint64_t s_key[kLCap];
+    __shared__ unsigned long long s_r0, s_r1;
+    const uint64_t w0 = (uint64_t)blockIdx.x * kLWin, w1 = min(w0 + kLWin, L.n);
+    const uint64_t lim = min(w1 + kLMax, L.n);
+    if (threadIdx.x == 0) {
+        s_r0 = w1;   // first group start in [w0, w1)
+        s_r1 = lim;  // first group start in [w1, lim), or lim when none (then the last group is long)
+    }
+    __syncthreads();
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // first group start at or after w0 (below w1) and after w1 (below lim):
+    // 256 positions per step, one atomic per wave, stop at the first step with one
+    for (uint64_t c0 = w0; c0 < w1; c0 += kLThreads) {
+        const uint64_t i = c0 + threadIdx.x;
+        const bool st = i < w1 && group_start(L.keys_in, i, L.shift);
+        const uint64_t m = __ballot(st);
+        if (lane == 0 && m) atomicMin(&s_r0, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
+        if (__syncthreads_or(st)) break;
+    }
+    for (uint64_t c0 = w1; c0 < lim; c0 += kLThreads) {
+        const uint64_t i = c0 + threadIdx.x;
+        const bool st = i < lim && group_start(L.keys_in, i, L.shift);
+        const uint64_t m = __ballot(st);
+        if (lane == 0 && m) atomicMin(&s_r1, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
+        if (__syncthreads_or(st)) break;
+    }
+    __syncthreads();
+    const uint64_t r0 = s_r0;
+    uint64_t r1 = s_r1;
+    if (r0 >= w1) return;  // no group starts here
+    if (r1 == lim && lim < L.n) {  // the last group of the window runs past w1 + kLMax: hand it over
+        __shared__ unsigned long long s_last;
+        if (threadIdx.x == 0) s_last = r0;
+        __syncthreads();
+        for (uint64_t c0 = r0; c0 < w1; c0 += kLThreads) {
+            const uint64_t i = c0 + threadIdx.x;
+            const uint64_t m = __ballot(i < w1 && group_start(L.keys_in, i, L.shift));
+            if (lane == 0 && m) atomicMax(&s_last, (unsigned long long)(c0 + wave * kWave + 63 - (uint64_t)__builtin_clzll(m)));
+        }
+        __syncthreads();
+        r1 = s_last;
+        if (threadIdx.x == 0) L.ovf[atomicAdd(L.novf, 1ull)] = s_last;
+    }
+    const uint32_t m = (uint32_t)(r1 - r0);
+    for (uint32_t j = threadIdx.x; j < m; j += kLThreads) s_key[j] = L.keys_in[r0 + j];
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < m; j += kLThreads) {
+        const uint64_t k = s_key[j], hk = k >> L.shift;
+        uint32_t gs = j, ge = j + 1;
+        while (gs > 0 && (s_key[gs - 1] >> L.shift) == hk) gs--;
+        while (ge < m && (s_key[ge] >> L.shift) == hk) ge++;
+        uint32_t rank = gs;
+        for (uint32_t t = gs; t < ge; t++) {
+            const uint64_t o = s_key[t];
+            rank += o < k || (o == k && t < j);
+        }
+        const uint64_t src = r0 + j, dst = r0 + rank;
+        L.keys_out[dst] = k;
+        L.lab_out[dst] = L.lab_in[src];
+        L.adj_out[dst] = L.adj_in[src];
+    }
+}
+
+// end of each long group: the next group start
+__global__ void long_end_kernel(const uint64_t *__restrict__ keys, uint64_t n, int shift,
+                                const uint64_t *__restrict__ starts, uint64_t nl, uint64_t *__restrict__ ends)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nl) return;
+    const uint64_t s = starts[g], hk = keys[s] >> shift;
+    uint64_t lo = s + 1, hi = n;  // first index > s with a different high part (sorted by it)
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((keys[mid] >> shift) == hk) lo = mid + 1; else hi = mid;
+    }
+    ends[g] = lo;
+}
+
+// Low bits ranked locally: the number of low key bits, 0 for the plain
+// all-global order.  About 30 high bits keep groups small; K = 32 keeps the
+// global order (the reference's forward code is 0 after the first base, so
+// most canonical k-mers are equal there).
+static int local_bits(int K)
+{
+    const int bits = 2 * K;
+    if (K >= 32 || bits < 40) return 0;
+    return ((bits - 30) / 8) * 8;
+}
+
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
 {
     // k-mer bytes only: within a run nothing downstream depends on the adj
     // order except the quirk's first tuples and the tuple dump, which derive it
-    return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FE, passes_out, nullptr, 0,
-                              nullptr);
+    const int S = getenv("MP_SORT_GLOBAL") ? 0 : local_bits(ctx->cfg.k);
+    if (S == 0 || n <= 1)
+        return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FE, passes_out, nullptr, 0,
+                                  nullptr);
+    uint32_t mask = 0;
+    for (int b = S / 8; b < 8; b++) mask |= 1u << (b + 1);
+    MP_TRY(radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, mask, passes_out, nullptr, 0,
+                              nullptr));
+    const uint64_t wins = (n + kLWin - 1) / kLWin;
+    MP_TRY(ensure(ctx, ctx->scratch, (wins + 2) * sizeof(uint64_t) + 64));
+    unsigned long long *novf = (unsigned long long *)ctx->scratch.p;
+    MP_TRY(fill_u32(ctx, novf, 0, 2));
+    const int src = ctx->cur, dst = 1 - ctx->cur;
+    LocalPass L;
+    L.keys_in = (const uint64_t *)ctx->keys[src].p;
+    L.lab_in = (const uint32_t *)ctx->lab[src].p;
+    L.adj_in = (const uint8_t *)ctx->adj[src].p;
+    L.keys_out = (uint64_t *)ctx->keys[dst].p;
+    L.lab_out = (uint32_t *)ctx->lab[dst].p;
+    L.adj_out = (uint8_t *)ctx->adj[dst].p;
+    L.n = n;
+    L.shift = S;
+    L.ovf = (uint64_t *)ctx->scratch.p + 1;
+    L.novf = novf;
+    hipEventRecord(ctx->pev[22], ctx->stream);
+    hipLaunchKernelGGL(local_rank_kernel, dim3((unsigned)wins), dim3(kLThreads), 0, ctx->stream, L);
+    MP_HIP(hipGetLastError());
+    hipEventRecord(ctx->pev[23], ctx->stream);
+    uint64_t nl = 0;
+    MP_HIP(hipMemcpyAsync(&nl, novf, sizeof(nl), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->pev[22], ctx->pev[23]);
+    ctx->local_ms = ms;
+    ctx->local_long = nl;
+    if (nl) {  // long groups: their own low-byte passes over [start, end)
+        std::vector<uint64_t> se(2 * nl);
+        uint64_t *d_ends = nullptr;
+        MP_HIP(hipMallocAsync((void **)&d_ends, nl * sizeof(uint64_t), ctx->stream));
+        hipLaunchKernelGGL(long_end_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream,
+                           L.keys_in, n, S, (const uint64_t *)L.ovf, nl, d_ends);
+        MP_HIP(hipGetLastError());
+        MP_HIP(hipMemcpyAsync(se.data(), L.ovf, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipMemcpyAsync(se.data() + nl, d_ends, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipFreeAsync(d_ends, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        uint32_t lmask = 0;
+        for (int b = 0; b < S / 8; b++) lmask |= 1u << (b + 1);
+        const float keep_avg = ctx->sort_pass_avg, keep_hist = ctx->sort_hist_ms;  // stats of the main passes
+        for (uint64_t g = 0; g < nl; g++) {
+            const uint64_t a = se[g], e = se[nl + g];
+            DevArray kv[2], lv[2], av[2];
+            for (int b = 0; b < 2; b++) {
+                kv[b].p = (uint64_t *)ctx->keys[b].p + a;
+                lv[b].p = (uint32_t *)ctx->lab[b].p + a;
+                av[b].p = (uint8_t *)ctx->adj[b].p + a;
+                kv[b].bytes = (e - a) * 8;
+                lv[b].bytes = (e - a) * 4;
+                av[b].bytes = e - a;
+            }
+            int c = src, np = 0;
+            MP_TRY(radix_sort_generic(ctx, kv, lv, av, &c, e - a, lmask, &np, nullptr, 0, nullptr));
+            if (c != dst) {  // an even number of passes ran: the range is still in the source buffer
+                MP_HIP(hipMemcpyAsync(kv[dst].p, kv[src].p, (e - a) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                MP_HIP(hipMemcpyAsync(lv[dst].p, lv[src].p, (e - a) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                MP_HIP(hipMemcpyAsync(av[dst].p, av[src].p, e - a, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        }
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->sort_pass_avg = keep_avg;
+        ctx->sort_hist_ms = keep_hist;
+    }
+    ctx->cur = dst;
+    *passes_out += 1;
+    return 0;
 }
 
 }  // namespace mp

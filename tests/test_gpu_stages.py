@@ -26,6 +26,12 @@ CASES = [
                            seed=8), 4, 31, 1, 1, False, False, 0),
 ]
 ORDERED = {"pairs_ordered"}
+CASES.append(
+    # one 3 kb genome at ~3000x: every k-mer has thousands of copies, so the
+    # sort's local pass hands whole groups to the per-group global passes
+    ("deep_repeat", dict(n_reads=60000, n_genomes=1, genome_len_min=3000, genome_len_max=3000, read_len=150,
+                         err_rate=0.005, seed=12), 4, 31, 1, 1, False, False, 0))
+LONG_GROUPS = {"deep_repeat"}
 
 
 @pytest.mark.gpu
@@ -49,6 +55,8 @@ def test_stages_match_oracle(case, tmp_path):
         for s in range(S):
             lo, hi = int(scan[s]), int(scan[s + 1])
             st = ctx.scan(s, lo, hi, thread[s * T:(s + 1) * T + 1], int(hist[min(hi, mp.NBUCKETS)] - hist[lo]))
+            if name in LONG_GROUPS:
+                assert st.sort_long_groups > 0
             if s == dump:
                 tup = ctx.dump_tuples()
                 assert tup.shape == ref["tuples"].shape

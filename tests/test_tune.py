@@ -1,0 +1,57 @@
+"""Host-side tuning and workload generators (CPU)."""
+import numpy as np
+
+import metapart_amd as mp
+from metapart_amd import tune
+
+
+def _prefix(counts):
+    H = np.zeros(mp.NBUCKETS + 1, dtype=np.uint64)
+    H[1:] = np.cumsum(counts)
+    return H
+
+
+def test_scans_for_budget_fits_and_is_minimal():
+    rng = np.random.default_rng(3)
+    counts = rng.poisson(11000, size=mp.NBUCKETS).astype(np.uint64)  # ~11.5 G k-mers
+    H = _prefix(counts)
+    reads, fq, bases = 100_000_000, 31_500_000_000, 15_000_000_000
+    prev = None
+    for budget in (400e9, 270e9, 200e9, 150e9):
+        S, need = tune.scans_for_budget(H, reads, fq, bases, int(budget))
+        assert need <= budget
+        if S > 1:  # one scan fewer does not fit (or cannot be planned)
+            try:
+                scan, _ = mp.plan_ranges(H, S - 1, 1)
+                hi = np.minimum(scan[1:], mp.NBUCKETS).astype(np.int64)
+                lo = np.minimum(scan[:-1], mp.NBUCKETS).astype(np.int64)
+                biggest = int((H[hi] - H[lo]).max())
+                base = tune.resident_bytes(reads, fq, bases, int(H[-1]))
+                assert base + biggest * tune.PER_TUPLE_DEVICE + max(0, 2 * int(H[-1]) - 8 * biggest) > budget
+            except mp.MPError:
+                pass
+        if prev is not None:
+            assert S >= prev  # less memory never needs fewer passes
+        prev = S
+
+
+def test_scans_for_budget_refuses_what_cannot_fit():
+    counts = np.zeros(mp.NBUCKETS, dtype=np.uint64)
+    counts[0] = 10 ** 9  # one bucket: no pass count splits it
+    try:
+        tune.scans_for_budget(_prefix(counts), 10 ** 7, 3 * 10 ** 9, 10 ** 9, 10 ** 9, max_scans=8)
+    except mp.MPError as e:
+        assert e.code == -2
+    else:
+        raise AssertionError("expected MP_ENOMEM")
+
+
+def test_workload_generators_are_seeded():
+    a, b = mp.mix_params(seed=4), mp.mix_params(seed=4)
+    assert a == b and len(a) == 64
+    assert all(x["n_reads"] % 2 == 0 and 10_000 <= x["n_reads"] <= 5_000_002 for x in a)
+    assert all(20 <= x["n_genomes"] <= 300 and 100 <= x["read_len"] <= 150 for x in a)
+    j = mp.job_params(seed=7)
+    assert j == mp.job_params(seed=7) and len(j) == 64
+    assert all(x["n_reads"] % 2 == 0 and 50_000 <= x["n_reads"] <= 2_000_002 and x["read_len"] == 150 for x in j)
+    assert len({x["seed"] for x in j}) == len(j)  # a fresh genome pool per job
