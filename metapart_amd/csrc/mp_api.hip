@@ -54,7 +54,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->rcnt, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -88,6 +88,7 @@ static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *c
     ctx->last_n = 0;
     ctx->batch = false;
     ctx->sample_W.clear();
+    ctx->rcnt_valid = false;
     MP_TRY(parse_reads(ctx, chunk_ofs, chunk_first_id, C));
     ctx->chunk_ofs.assign(chunk_ofs, chunk_ofs + C + 1);
     const uint64_t N = ctx->N;
@@ -244,7 +245,8 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     }
     hipEventRecord(ctx->ev[0], ctx->stream);
     uint64_t n = 0;
-    MP_TRY(enumerate_scan(ctx, s, lo, hi, &n));
+    const bool every = !ctx->cfg.min3 && ctx->rcnt_valid && lo == 0 && expected_tuples == ctx->kmer_total;
+    MP_TRY(enumerate_scan(ctx, s, lo, hi, &n, every));
     hipEventRecord(ctx->ev[1], ctx->stream);
     if (expected_tuples && n != expected_tuples)
         return ctx->fail(MP_ECOUNT, "scan %d enumerated %llu tuples, the histogram sized %llu", s,

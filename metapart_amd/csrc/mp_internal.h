@@ -108,6 +108,9 @@ struct mp_ctx {
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
+    mp::DevArray rcnt;    // k-mers per read from the last m-mer histogram (u32)
+    bool rcnt_valid = false;
+    uint64_t kmer_total = 0;
     // writer
     mp::DevArray out, items[2], ilens[2], iaux[2];
     // pinned host staging of large uploads (two buffers, one copy in flight)
@@ -168,7 +171,9 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
 // per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
 int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
+// every_kmer: the scan's range holds every k-mer and the per-read counts of
+// the last m-mer histogram are valid for the loaded reads (skips a count pass)
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool every_kmer = false);
 int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out);
 // Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:
 // 0 = adj byte, 1..8 = key byte p-1, 9 = sample of the label (d_grp_first:

@@ -212,6 +212,7 @@ constexpr uint32_t kHReads = 1024;       // reads per tile (steps 1, 2)
 constexpr uint64_t kHSlab = 4u << 20;    // items per workgroup (step 3)
 
 struct HParams {
+    uint32_t *rcnt;             // k-mers of every read (counting step)
     const uint64_t *chunk_ids;  // C+1 first read ids
     const uint32_t *tile0;      // C+1 first tile of each chunk
     int C;
@@ -233,6 +234,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint64_t s_base[kRG];
     __shared__ uint32_t s_dig[kHDigits];
     __shared__ uint64_t s_out[kHDigits];
+    __shared__ uint32_t s_rc[kRG];
     const int tid = threadIdx.x;
     const uint32_t t = blockIdx.x;
     int c = 0, hi = H.C - 1;  // chunk: last c with tile0[c] <= t
@@ -253,7 +255,10 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         __syncthreads();  // previous group done with s_pref / s_base
         const uint64_t p0 = P.pos_off[r0];
         if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
-        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        if (tid < nr) {
+            s_base[tid] = P.base_off[r0 + tid];
+            s_rc[tid] = 0;
+        }
         __syncthreads();
         const uint32_t total = s_pref[nr];
         for (uint32_t q = tid; q < total; q += kKBlock) {
@@ -265,11 +270,18 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             const uint32_t p = q - s_pref[lo];
             uint64_t canon;
             bool fwd;
-            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) {
+                if (PHASE == 0) atomicAdd(&s_rc[lo], 1u);  // windows with an 'N'
+                continue;
+            }
             const uint32_t b = (uint32_t)(canon >> P.bshift);
             const uint32_t d = b >> kHLowBits;
             const uint32_t slot = atomicAdd(&s_dig[d], 1u);
             if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
+        }
+        if (PHASE == 0) {  // k-mers per read, reused by the enumeration of a scan over every bucket
+            __syncthreads();
+            if (tid < nr) H.rcnt[r0 + tid] = (s_pref[tid + 1] - s_pref[tid] - s_rc[tid]);
         }
     }
     if (PHASE == 0) {
@@ -341,6 +353,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_TRY(ensure(ctx, ctx->wg_off, (M + 1) * sizeof(uint64_t)));
     uint8_t *hb = (uint8_t *)ctx->hpart.p;
     HParams H{};
+    MP_TRY(ensure(ctx, ctx->rcnt, ctx->N * sizeof(uint32_t) + 64));
+    H.rcnt = (uint32_t *)ctx->rcnt.p;
     H.chunk_ids = (const uint64_t *)(hb + o_ids);
     H.tile0 = (const uint32_t *)(hb + o_t0);
     H.C = C;
@@ -359,6 +373,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_HIP(hipMemcpyAsync(bounds.data(), d_bounds, 8 * (nparts + 1), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
     const uint64_t total = bounds[nparts];
+    ctx->rcnt_valid = true;
+    ctx->kmer_total = total;
     if (total == 0) return 0;
     // the low bits stage in the second tuple buffer, unused until the first scan's sort
     MP_TRY(ensure(ctx, ctx->keys[1], total * sizeof(uint16_t) + 64));
@@ -593,7 +609,19 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
     return 0;
 }
 
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out)
+// k-mers of the reads of each enumeration workgroup (kRG consecutive reads)
+__global__ void group_sum_kernel(const uint32_t *__restrict__ rcnt, uint64_t N, uint32_t *__restrict__ out,
+                                 uint64_t nwg)
+{
+    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwg) return;
+    uint32_t t = 0;
+    const uint64_t e = min((w + 1) * kRG, N);
+    for (uint64_t r = w * kRG; r < e; r++) t += rcnt[r];
+    out[w] = t;
+}
+
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool every_kmer)
 {
     KParams P = base_params(ctx);
     P.lo = lo;
@@ -609,6 +637,9 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
     P.wg_cnt = (uint32_t *)ctx->wg_cnt.p;
     P.wg_off = (const uint64_t *)ctx->wg_off.p;
     if (m3) hipLaunchKernelGGL(kmer_min3_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
+    else if (every_kmer)  // the histogram counted every read's k-mers already
+        hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const uint32_t *)ctx->rcnt.p, ctx->N, (uint32_t *)ctx->wg_cnt.p, nwg);
     else hipLaunchKernelGGL(kmer_plain_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
     MP_HIP(hipGetLastError());
     MP_TRY(scan_u32(ctx, (const uint32_t *)ctx->wg_cnt.p, (uint64_t *)ctx->wg_off.p, nwg));
