@@ -403,6 +403,7 @@ __device__ __forceinline__ bool group_start(const uint64_t *keys, uint64_t i, in
 __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
 {
     __shared__ uint64_t s_key[kLCap];
+    __shared__ uint64_t s_st[kLCap / kWave];
     __shared__ unsigned long long s_r0, s_r1;
     const uint64_t w0 = (uint64_t)blockIdx.x * kLWin, w1 = min(w0 + kLWin, L.n);
     const uint64_t lim = min(w1 + kLMax, L.n);
@@ -446,20 +447,56 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         if (threadIdx.x == 0) L.ovf[atomicAdd(L.novf, 1ull)] = s_last;
     }
     const uint32_t m = (uint32_t)(r1 - r0);
-    for (uint32_t j = threadIdx.x; j < m; j += kLThreads) s_key[j] = L.keys_in[r0 + j];
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kLThreads) {
-        const uint64_t k = s_key[j], hk = k >> L.shift;
-        uint32_t gs = j, ge = j + 1;
-        while (gs > 0 && (s_key[gs - 1] >> L.shift) == hk) gs--;
-        while (ge < m && (s_key[ge] >> L.shift) == hk) ge++;
-        uint32_t rank = gs;
-        for (uint32_t t = gs; t < ge; t++) {
-            const uint64_t o = s_key[t];
-            rank += o < k || (o == k && t < j);
+    constexpr int kQ = kLCap / kLThreads;
+    uint64_t kr[kQ], c[kQ];  // keys, and (low bits << 12) | index: the order within a group, ties by position
+    const uint64_t lowmask = (1ull << L.shift) - 1;
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+        const uint32_t j = q * kLThreads + threadIdx.x;
+        if (j < m) {
+            kr[q] = L.keys_in[r0 + j];
+            s_key[j] = kr[q];
+            c[q] = ((kr[q] & lowmask) << 12) | j;
         }
+    }
+    __syncthreads();
+    // group starts as a bitmap: word w holds positions 64w .. 64w+63 (one wave's positions)
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+        const uint32_t j = q * kLThreads + threadIdx.x;
+        if (q * kLThreads >= m) break;
+        const bool st = j < m && (j == 0 || (s_key[j] >> L.shift) != (s_key[j - 1] >> L.shift));
+        const uint64_t b = __ballot(st);
+        if (lane == 0) s_st[j / kWave] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+        const uint32_t j = q * kLThreads + threadIdx.x;
+        if (j < m) s_key[j] = c[q];
+    }
+    __syncthreads();
+    const uint32_t nw = (m + kWave - 1) / kWave;
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+        const uint32_t j = q * kLThreads + threadIdx.x;
+        if (q * kLThreads >= m) break;
+        if (j >= m) continue;
+        uint32_t w = j / kWave;
+        const uint32_t bit = j % kWave;
+        const uint64_t upto = bit == 63 ? ~0ull : (2ull << bit) - 1;
+        uint64_t bs = s_st[w] & upto;  // position 0 is a start: the search ends
+        while (!bs) bs = s_st[--w];
+        const uint32_t gs = w * kWave + 63 - (uint32_t)__builtin_clzll(bs);
+        w = j / kWave;
+        uint64_t be = s_st[w] & ~upto;
+        while (!be && ++w < nw) be = s_st[w];
+        const uint32_t ge = be ? w * kWave + (uint32_t)__builtin_ctzll(be) : m;
+        const uint64_t cj = c[q];
+        uint32_t rank = gs;
+        for (uint32_t t = gs; t < ge; t++) rank += s_key[t] < cj;
         const uint64_t src = r0 + j, dst = r0 + rank;
-        L.keys_out[dst] = k;
+        L.keys_out[dst] = kr[q];
         L.lab_out[dst] = L.lab_in[src];
         L.adj_out[dst] = L.adj_in[src];
     }
