@@ -1,11 +1,12 @@
 // C-ABI entry points of libmetapart_amd (declared in include/metapart_amd.h).
+#include <algorithm>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "mp_internal.h"
-
-#include <algorithm>
-#include <thread>
+#include "mp_kmer.h"
 
 using namespace mp;
 
@@ -54,7 +55,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->rcnt, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->khist, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -88,7 +89,7 @@ static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *c
     ctx->last_n = 0;
     ctx->batch = false;
     ctx->sample_W.clear();
-    ctx->rcnt_valid = false;
+    ctx->khist_valid = false;
     MP_TRY(parse_reads(ctx, chunk_ofs, chunk_first_id, C));
     ctx->chunk_ofs.assign(chunk_ofs, chunk_ofs + C + 1);
     const uint64_t N = ctx->N;
@@ -245,14 +246,22 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     }
     hipEventRecord(ctx->ev[0], ctx->stream);
     uint64_t n = 0;
-    const bool every = !ctx->cfg.min3 && ctx->rcnt_valid && lo == 0 && expected_tuples == ctx->kmer_total;
-    MP_TRY(enumerate_scan(ctx, s, lo, hi, &n, every));
+    // every k-mer: the first radix pass generates the tuples from the reads;
+    // 3-minimizer k-mers (a per-read window state machine) are enumerated
+    std::unique_ptr<GenSource> gen;
+    if (ctx->cfg.min3) {
+        MP_TRY(enumerate_scan(ctx, s, lo, hi, &n));
+    } else {
+        gen.reset(new GenSource());
+        const bool all = lo == 0 && expected_tuples != 0 && expected_tuples == ctx->kmer_total;
+        MP_TRY(gen_histograms(ctx, s, lo, hi, scan_pass_mask(ctx->cfg.k), all, gen.get(), &n));
+    }
     hipEventRecord(ctx->ev[1], ctx->stream);
     if (expected_tuples && n != expected_tuples)
         return ctx->fail(MP_ECOUNT, "scan %d enumerated %llu tuples, the histogram sized %llu", s,
                          (unsigned long long)n, (unsigned long long)expected_tuples);
     int passes = 0;
-    MP_TRY(radix_sort_tuples(ctx, n, &passes));
+    MP_TRY(radix_sort_tuples(ctx, n, &passes, gen.get()));
     hipEventRecord(ctx->ev[7], ctx->stream);
     MP_TRY(filter_union_scan(ctx, n, seg_bounds, T, &st));
     st.tuples = n;
@@ -261,6 +270,7 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     st.ms_sort_hist = ctx->sort_hist_ms;
     st.ms_sort_local = ctx->local_ms;
     st.sort_long_groups = ctx->local_long;
+    st.ms_sort_gen = gen ? ctx->sort_gen_ms : 0.f;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     if (stats) *stats = st;
@@ -410,7 +420,13 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     ctx->scans_done = 0;
     hipEventRecord(ctx->ev[0], ctx->stream);
     uint64_t n = 0;
-    MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n));
+    std::unique_ptr<GenSource> gen;
+    if (ctx->cfg.min3) {
+        MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n));
+    } else {
+        gen.reset(new GenSource());
+        MP_TRY(gen_histograms(ctx, 0, 0, MP_NBUCKETS, scan_pass_mask(ctx->cfg.k), true, gen.get(), &n));
+    }
     hipEventRecord(ctx->ev[1], ctx->stream);
     // sort by (sample, k-mer, adj): the sample of a label is its chunk
     uint64_t *d_first = nullptr;
@@ -419,7 +435,8 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
                           ctx->stream));
     std::vector<uint64_t> tpos(ns + 1, 0);
     int passes = 0;
-    int rc = radix_sort_tuples(ctx, n, &passes);
+    int rc = radix_sort_tuples(ctx, n, &passes, gen.get());
+    const float gen_ms = gen ? ctx->sort_gen_ms : 0.f;
     if (!rc) {  // then the sample of the label as the most significant digit
         int sp = 0;
         rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 1u << 9, &sp, d_first, ns,
@@ -453,6 +470,7 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     st.ms_sort_hist = ctx->sort_hist_ms;
     st.ms_sort_local = ctx->local_ms;
     st.sort_long_groups = ctx->local_long;
+    st.ms_sort_gen = gen_ms;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     ctx->last_n = n;

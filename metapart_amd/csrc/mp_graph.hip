@@ -286,9 +286,24 @@ __global__ void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits, co
         while (c < e && is_bad(badbits, G.lab[c])) c++;
         if (c >= e) continue;
         const uint32_t a = G.lab[c];
+        // star around a, keeping a's root between members: a stale root is
+        // still a valid hook target (it only grows towards the real root)
+        uint32_t ra = uf_find(G.parent, a);
         for (uint64_t t = c + 1; t < e; t++) {
             const uint32_t b = G.lab[t];
-            if (b != a && !is_bad(badbits, b)) uf_union(G.parent, a, b);
+            if (b == a || is_bad(badbits, b)) continue;
+            uint32_t rb = uf_find(G.parent, b);
+            while (ra != rb) {
+                const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
+                uint32_t expect = lo;  // hook the smaller root under the larger (unionfind.h:58-69)
+                if (__hip_atomic_compare_exchange_strong(&G.parent[lo], &expect, hi, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ra = hi;
+                    break;
+                }
+                ra = uf_find(G.parent, ra);
+                rb = uf_find(G.parent, rb);
+            }
         }
     }
 }

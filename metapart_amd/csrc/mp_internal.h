@@ -14,6 +14,8 @@ namespace mp {
 
 constexpr int kWave = 64;
 constexpr uint32_t kBad = 1;  // bit of bad[]: label marked by the frequency filter
+constexpr int kSortRadix = 256;
+constexpr int kSortPasses = 10;  // radix passes: adj, k-mer bytes 0..7, sample of the label
 
 // ---------------------------------------------------------------- device helpers
 __device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
@@ -82,6 +84,7 @@ struct mp_ctx {
     hipEvent_t ev[8]{};
     hipEvent_t pev[24]{};  // per radix pass (timing of the dominant kernel)
     float sort_pass_avg = 0, sort_hist_ms = 0;
+    float sort_gen_ms = 0;     // the generating radix pass of the last sort (0: none)
     float local_ms = 0;        // local rank pass of the last sort
     uint64_t local_long = 0;   // groups it handed to the long-group kernel
     std::string err;
@@ -108,9 +111,10 @@ struct mp_ctx {
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
-    mp::DevArray rcnt;    // k-mers per read from the last m-mer histogram (u32)
-    bool rcnt_valid = false;
-    uint64_t kmer_total = 0;
+    mp::DevArray khist;   // m-mer histogram: counts of the k-mer bytes the scans' radix passes read
+    bool khist_valid = false;
+    uint32_t khist_mask = 0;  // passes (bit 1 + byte) khist counts
+    uint64_t kmer_total = 0;  // k-mers of the loaded reads (last m-mer histogram)
     // writer
     mp::DevArray out, items[2], ilens[2], iaux[2];
     // pinned host staging of large uploads (two buffers, one copy in flight)
@@ -171,17 +175,27 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
 // per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
 int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
-// every_kmer: the scan's range holds every k-mer and the per-read counts of
-// the last m-mer histogram are valid for the loaded reads (skips a count pass)
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool every_kmer = false);
-int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out);
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
+// The scan's tuples generated by the first radix pass instead of an
+// enumeration: the reads' k-mers in [lo, hi) and their digit histograms.
+struct GenSource;
+// passes whose digits the sort of a scan's tuples reads (bit p = radix pass p)
+uint32_t scan_pass_mask(int K);
+// kSortPasses x kSortRadix digit counts of the scan's tuples (host, passes of
+// pass_mask) and their number; fills *g for radix_sort_tuples.  all: the scan
+// holds every k-mer, so the counts of the last m-mer histogram serve.
+int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_mask, bool all, GenSource *g,
+                   uint64_t *n_out);
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen = nullptr);
 // Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:
 // 0 = adj byte, 1..8 = key byte p-1, 9 = sample of the label (d_grp_first:
 // n_grp + 1 first read ids on the device; grp_base, if given, receives the
 // n_grp + 1 tuple offsets of the samples).  *cur selects the buffer holding the data.
+// With gen, the first pass generates the tuples (gen's histograms replace the
+// histogram read) into the buffer *cur does not select.
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base);
+                       uint64_t *grp_base, const GenSource *gen = nullptr);
 int iota_u32(mp_ctx *ctx, uint32_t *p, uint64_t n);
 int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st);
 // Same over segments at the given tuple positions (nseg + 1, host) with

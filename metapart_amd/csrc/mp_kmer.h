@@ -1,0 +1,98 @@
+// Device-side k-mer generation shared by the enumeration kernels and the
+// radix pass that generates its input (canonical k-mer, adj byte, label).
+#pragma once
+#include "mp_internal.h"
+
+namespace mp {
+
+constexpr int kRG = 64;       // reads per workgroup, plain kernels
+constexpr int kKBlock = 256;  // lanes per workgroup
+
+struct KParams {
+    const uint8_t *fq;
+    const uint64_t *seq_off;
+    const uint32_t *seq_len;
+    const uint64_t *base_off;
+    const uint64_t *pos_off;
+    const uint64_t *packed;
+    const uint64_t *nmask;
+    uint64_t N;
+    int K;
+    int bshift;
+    uint64_t lo, hi;
+    // histogram
+    uint32_t *counts;
+    const uint64_t *chunk_ids;
+    int C;
+    // count / emit
+    uint32_t *wg_cnt;
+    const uint64_t *wg_off;
+    uint64_t *keys;
+    uint8_t *adj;
+    uint32_t *lab;
+    const uint32_t *parent;
+    const uint8_t *bad;
+    int scan0;
+};
+
+enum { kModeHist = 0, kModeCount = 1, kModeEmit = 2 };
+
+__device__ __forceinline__ uint8_t rc_char(uint8_t c)
+{
+    // "TGAC   N"[(c >> 1) & 7]  (enumerateKmers.c:237)
+    return (uint8_t)((0x4E20202043414754ull >> (8 * ((c >> 1) & 7))) & 0xFF);
+}
+__device__ __forceinline__ uint8_t adj_of(uint8_t prev, uint8_t next)
+{
+    return (uint8_t)(((prev << 2) & 0xF8) | ((next >> 1) & 0x7));
+}
+__device__ __forceinline__ uint32_t code_of(uint8_t c) { return (0x2310u >> (((c & 6) >> 1) * 4)) & 3; }
+
+// Canonical k-mer of the window starting at base b (position p of a read with
+// sequence at fq[so], length L).  Returns false if the window holds an 'N'.
+__device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
+{
+    const int K = P.K;
+    {
+        const uint64_t w = b >> 6;
+        const int o = (int)(b & 63);
+        const uint64_t hi = P.nmask[w], lo = P.nmask[w + 1];
+        const int s = o + K;
+        const uint64_t v = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
+        if (v & ((1ull << K) - 1)) return false;
+    }
+    const uint64_t w = b >> 5;
+    const int o = (int)(b & 31);
+    const uint64_t hi = P.packed[w], lo = P.packed[w + 1];
+    const int s = 2 * (o + K);
+    uint64_t F = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
+    if (K < 32) F &= (1ull << (2 * K)) - 1;
+    const uint64_t R = revcomp(F, K);
+    if (K == 32 && p > 0) F = 0;
+    fwd = F < R;
+    canon = fwd ? F : R;
+    return true;
+}
+
+// adj byte of the window at p of a read (sequence at fq[so], length L)
+__device__ __forceinline__ uint8_t adj_at(const KParams &P, uint64_t so, uint32_t L, uint32_t p, bool fwd)
+{
+    const uint8_t prev = p == 0 ? (uint8_t)'N' : P.fq[so + p - 1];
+    const uint8_t next = p == L - (uint32_t)P.K ? (uint8_t)'N' : P.fq[so + p + P.K];
+    return fwd ? adj_of(prev, next) : adj_of(rc_char(next), rc_char(prev));
+}
+
+// label of read r: the read itself in the first scan; later scans label a
+// read with its component root, or with itself once the filter marked it
+__device__ __forceinline__ uint32_t label_of(const KParams &P, uint64_t r)
+{
+    return P.scan0 ? (uint32_t)r : (P.bad[r] ? (uint32_t)r : P.parent[r]);
+}
+
+struct GenSource {
+    KParams P;                                // reads, K, bucket range, labels
+    uint64_t h[kSortPasses * kSortRadix];     // digit counts of the tuples
+    uint64_t npos;                            // k-mer windows of all reads
+};
+
+}  // namespace mp

@@ -21,80 +21,13 @@
 // enumerateKmers_3minimizers.c:129-474): a sequential window state machine per
 // read, one lane per read, over the same O(1) canonical k-mers.
 #include "mp_internal.h"
+#include "mp_kmer.h"
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 namespace mp {
-
-constexpr int kRG = 64;       // reads per workgroup, plain kernels
-constexpr int kKBlock = 256;  // lanes per workgroup
-
-struct KParams {
-    const uint8_t *fq;
-    const uint64_t *seq_off;
-    const uint32_t *seq_len;
-    const uint64_t *base_off;
-    const uint64_t *pos_off;
-    const uint64_t *packed;
-    const uint64_t *nmask;
-    uint64_t N;
-    int K;
-    int bshift;
-    uint64_t lo, hi;
-    // histogram
-    uint32_t *counts;
-    const uint64_t *chunk_ids;
-    int C;
-    // count / emit
-    uint32_t *wg_cnt;
-    const uint64_t *wg_off;
-    uint64_t *keys;
-    uint8_t *adj;
-    uint32_t *lab;
-    const uint32_t *parent;
-    const uint8_t *bad;
-    int scan0;
-};
-
-enum { kModeHist = 0, kModeCount = 1, kModeEmit = 2 };
-
-__device__ __forceinline__ uint8_t rc_char(uint8_t c)
-{
-    // "TGAC   N"[(c >> 1) & 7]  (enumerateKmers.c:237)
-    return (uint8_t)((0x4E20202043414754ull >> (8 * ((c >> 1) & 7))) & 0xFF);
-}
-__device__ __forceinline__ uint8_t adj_of(uint8_t prev, uint8_t next)
-{
-    return (uint8_t)(((prev << 2) & 0xF8) | ((next >> 1) & 0x7));
-}
-__device__ __forceinline__ uint32_t code_of(uint8_t c) { return (0x2310u >> (((c & 6) >> 1) * 4)) & 3; }
-
-// Canonical k-mer of the window starting at base b (position p of a read with
-// sequence at fq[so], length L).  Returns false if the window holds an 'N'.
-__device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
-{
-    const int K = P.K;
-    {
-        const uint64_t w = b >> 6;
-        const int o = (int)(b & 63);
-        const uint64_t hi = P.nmask[w], lo = P.nmask[w + 1];
-        const int s = o + K;
-        const uint64_t v = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
-        if (v & ((1ull << K) - 1)) return false;
-    }
-    const uint64_t w = b >> 5;
-    const int o = (int)(b & 31);
-    const uint64_t hi = P.packed[w], lo = P.packed[w + 1];
-    const int s = 2 * (o + K);
-    uint64_t F = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
-    if (K < 32) F &= (1ull << (2 * K)) - 1;
-    const uint64_t R = revcomp(F, K);
-    if (K == 32 && p > 0) F = 0;
-    fwd = F < R;
-    canon = fwd ? F : R;
-    return true;
-}
 
 template <int MODE>
 __global__ __launch_bounds__(kKBlock) void kmer_plain_kernel(KParams P)
@@ -211,8 +144,11 @@ constexpr int kHLow = 1 << kHLowBits;
 constexpr uint32_t kHReads = 1024;       // reads per tile (steps 1, 2)
 constexpr uint64_t kHSlab = 4u << 20;    // items per workgroup (step 3)
 
+constexpr int kBhRep = 32;  // copies of the byte histograms (tile t adds to copy t % kBhRep)
+
 struct HParams {
-    uint32_t *rcnt;             // k-mers of every read (counting step)
+    unsigned long long *bh;     // kBhRep x 8 x 256 counts of the k-mer bytes in bmask
+    uint32_t bmask;             // bit 1 + b: count byte b (the radix passes of the scans)
     const uint64_t *chunk_ids;  // C+1 first read ids
     const uint32_t *tile0;      // C+1 first tile of each chunk
     int C;
@@ -234,8 +170,10 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint64_t s_base[kRG];
     __shared__ uint32_t s_dig[kHDigits];
     __shared__ uint64_t s_out[kHDigits];
-    __shared__ uint32_t s_rc[kRG];
+    __shared__ uint32_t s_bh[PHASE == 0 ? 8 : 1][kSortRadix];
     const int tid = threadIdx.x;
+    if (PHASE == 0)
+        for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&s_bh[0][0])[j] = 0;
     const uint32_t t = blockIdx.x;
     int c = 0, hi = H.C - 1;  // chunk: last c with tile0[c] <= t
     while (c < hi) {
@@ -255,10 +193,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         __syncthreads();  // previous group done with s_pref / s_base
         const uint64_t p0 = P.pos_off[r0];
         if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
-        if (tid < nr) {
-            s_base[tid] = P.base_off[r0 + tid];
-            s_rc[tid] = 0;
-        }
+        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
         __syncthreads();
         const uint32_t total = s_pref[nr];
         for (uint32_t q = tid; q < total; q += kKBlock) {
@@ -270,23 +205,26 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             const uint32_t p = q - s_pref[lo];
             uint64_t canon;
             bool fwd;
-            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) {
-                if (PHASE == 0) atomicAdd(&s_rc[lo], 1u);  // windows with an 'N'
-                continue;
-            }
+            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
             const uint32_t b = (uint32_t)(canon >> P.bshift);
             const uint32_t d = b >> kHLowBits;
             const uint32_t slot = atomicAdd(&s_dig[d], 1u);
             if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
-        }
-        if (PHASE == 0) {  // k-mers per read, reused by the enumeration of a scan over every bucket
-            __syncthreads();
-            if (tid < nr) H.rcnt[r0 + tid] = (s_pref[tid + 1] - s_pref[tid] - s_rc[tid]);
+            if (PHASE == 0) {  // digit counts of a later sort over every k-mer
+#pragma unroll
+                for (int y = 0; y < 8; y++)
+                    if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
+            }
         }
     }
     if (PHASE == 0) {
         __syncthreads();
         if (tid < kHDigits) H.tcnt[mbase + (uint64_t)tid * Tc] = s_dig[tid];
+        unsigned long long *bh = H.bh + (uint64_t)(t % kBhRep) * 8 * kSortRadix;
+        for (int j = tid; j < 8 * kSortRadix; j += kKBlock) {
+            const uint32_t v = (&s_bh[0][0])[j];
+            if (v) atomicAdd(&bh[j], (unsigned long long)v);
+        }
     }
 }
 
@@ -353,8 +291,10 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_TRY(ensure(ctx, ctx->wg_off, (M + 1) * sizeof(uint64_t)));
     uint8_t *hb = (uint8_t *)ctx->hpart.p;
     HParams H{};
-    MP_TRY(ensure(ctx, ctx->rcnt, ctx->N * sizeof(uint32_t) + 64));
-    H.rcnt = (uint32_t *)ctx->rcnt.p;
+    MP_TRY(ensure(ctx, ctx->khist, sizeof(uint64_t) * kBhRep * 8 * kSortRadix));
+    MP_TRY(fill_u32(ctx, ctx->khist.p, 0, 2 * kBhRep * 8 * kSortRadix));
+    H.bh = (unsigned long long *)ctx->khist.p;
+    H.bmask = scan_pass_mask(ctx->cfg.k) & 0x1FEu;
     H.chunk_ids = (const uint64_t *)(hb + o_ids);
     H.tile0 = (const uint32_t *)(hb + o_t0);
     H.C = C;
@@ -373,7 +313,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_HIP(hipMemcpyAsync(bounds.data(), d_bounds, 8 * (nparts + 1), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
     const uint64_t total = bounds[nparts];
-    ctx->rcnt_valid = true;
+    ctx->khist_mask = H.bmask;
+    ctx->khist_valid = true;
     ctx->kmer_total = total;
     if (total == 0) return 0;
     // the low bits stage in the second tuple buffer, unused until the first scan's sort
@@ -609,19 +550,99 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
     return 0;
 }
 
-// k-mers of the reads of each enumeration workgroup (kRG consecutive reads)
-__global__ void group_sum_kernel(const uint32_t *__restrict__ rcnt, uint64_t N, uint32_t *__restrict__ out,
-                                 uint64_t nwg)
+// Digit counts of a scan's tuples for the radix sort whose first pass
+// generates them: every in-range k-mer of every read, counted per k-mer byte
+// of the pass mask in LDS (groups of kRG reads, grid-stride), then added to
+// ghist[pass][digit] (pass 1 + b for byte b).
+__global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uint32_t pass_mask,
+                                                                  unsigned long long *__restrict__ ghist)
 {
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwg) return;
-    uint32_t t = 0;
-    const uint64_t e = min((w + 1) * kRG, N);
-    for (uint64_t r = w * kRG; r < e; r++) t += rcnt[r];
-    out[w] = t;
+    __shared__ uint32_t h[8][kSortRadix];
+    __shared__ uint32_t s_pref[kRG + 1];
+    __shared__ uint64_t s_base[kRG];
+    const int tid = threadIdx.x;
+    for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&h[0][0])[j] = 0;
+    const uint64_t ngrp = (P.N + kRG - 1) / kRG;
+    for (uint64_t g = blockIdx.x; g < ngrp; g += gridDim.x) {
+        const uint64_t r0 = g * kRG;
+        const int nr = (int)((P.N - r0) < (uint64_t)kRG ? (P.N - r0) : (uint64_t)kRG);
+        __syncthreads();  // the previous group is done with s_pref / s_base
+        const uint64_t p0 = P.pos_off[r0];
+        if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
+        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        __syncthreads();
+        const uint32_t total = s_pref[nr];
+        for (uint32_t q = tid; q < total; q += kKBlock) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pref[mid] <= q) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t p = q - s_pref[lo];
+            uint64_t canon;
+            bool fwd;
+            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            const uint64_t bk = canon >> P.bshift;
+            if (bk < P.lo || bk >= P.hi) continue;
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                if ((pass_mask >> (b + 1)) & 1) atomicAdd(&h[b][(canon >> (8 * b)) & 0xFF], 1u);
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < 8 * kSortRadix; j += kKBlock) {
+        const uint32_t v = (&h[0][0])[j];
+        if (v) atomicAdd(&ghist[kSortRadix + j], (unsigned long long)v);
+    }
 }
 
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool every_kmer)
+int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_mask, bool all, GenSource *g,
+                   uint64_t *n_out)
+{
+    if (!pass_mask || (pass_mask & ~0x1FEu)) return ctx->fail(MP_EINVAL, "generated sorts take k-mer byte passes only");
+    KParams P = base_params(ctx);
+    P.lo = lo;
+    P.hi = hi;
+    P.scan0 = s == 0;
+    P.parent = (const uint32_t *)ctx->parent.p;
+    P.bad = (const uint8_t *)ctx->bad.p;
+    MP_HIP(hipMemcpyAsync(&g->npos, (const uint64_t *)ctx->pos_off.p + ctx->N, sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (all && ctx->khist_valid && !(pass_mask & ~ctx->khist_mask)) {
+        // every k-mer is in the scan: the m-mer histogram counted these digits already
+        std::vector<uint64_t> rep((size_t)kBhRep * 8 * kSortRadix);
+        MP_HIP(hipMemcpyAsync(rep.data(), ctx->khist.p, rep.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        memset(g->h, 0, sizeof(g->h));
+        for (int c = 0; c < kBhRep; c++)
+            for (int j = 0; j < 8 * kSortRadix; j++) g->h[kSortRadix + j] += rep[(size_t)c * 8 * kSortRadix + j];
+    } else {
+        MP_TRY(ensure(ctx, ctx->hist, sizeof(uint64_t) * (kSortPasses * kSortRadix + 64)));
+        unsigned long long *gh = (unsigned long long *)ctx->hist.p;
+        MP_TRY(fill_u32(ctx, gh, 0, 2 * kSortPasses * kSortRadix));
+        const uint64_t ngrp = (ctx->N + kRG - 1) / kRG;
+        const unsigned blocks = (unsigned)(ngrp < 4096 ? ngrp : 4096);
+        if (blocks) {
+            hipLaunchKernelGGL(kmer_digit_hist_kernel, dim3(blocks), dim3(kKBlock), 0, ctx->stream, P, pass_mask, gh);
+            MP_HIP(hipGetLastError());
+        }
+        MP_HIP(hipMemcpyAsync(g->h, gh, sizeof(g->h), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    uint64_t n = 0;
+    const int b0 = __builtin_ctz(pass_mask);
+    for (int r = 0; r < kSortRadix; r++) n += g->h[b0 * kSortRadix + r];
+    for (int b = 0; b < 2; b++) {
+        MP_TRY(ensure(ctx, ctx->keys[b], (n + 1) * sizeof(uint64_t)));
+        MP_TRY(ensure(ctx, ctx->lab[b], (n + 1) * sizeof(uint32_t)));
+        MP_TRY(ensure(ctx, ctx->adj[b], (n + 16) * sizeof(uint8_t)));
+    }
+    g->P = P;
+    *n_out = n;
+    return 0;
+}
+
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out)
 {
     KParams P = base_params(ctx);
     P.lo = lo;
@@ -637,9 +658,6 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
     P.wg_cnt = (uint32_t *)ctx->wg_cnt.p;
     P.wg_off = (const uint64_t *)ctx->wg_off.p;
     if (m3) hipLaunchKernelGGL(kmer_min3_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
-    else if (every_kmer)  // the histogram counted every read's k-mers already
-        hipLaunchKernelGGL(group_sum_kernel, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const uint32_t *)ctx->rcnt.p, ctx->N, (uint32_t *)ctx->wg_cnt.p, nwg);
     else hipLaunchKernelGGL(kmer_plain_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
     MP_HIP(hipGetLastError());
     MP_TRY(scan_u32(ctx, (const uint32_t *)ctx->wg_cnt.p, (uint64_t *)ctx->wg_off.p, nwg));

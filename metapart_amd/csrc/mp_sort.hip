@@ -21,7 +21,14 @@
 //   - per-digit tile counts (from the per-wave counters of the ranking) are
 //     published and the digit's global prefix is found by decoupled look-back
 //     over earlier tiles (epoch-tagged status words, so the status array is
-//     never cleared between passes),
+//     never cleared between passes).  With ~500 tiles in flight a walk
+//     passes tens of tiles that have only their own count published, and
+//     each step is a device-coherent load from another CU (often another
+//     XCD), so the walk reads 4 predecessors per step with their loads in
+//     flight together (measured at 2.4e9 tuples: 21.5 ms per pass against
+//     23.0 for one, 23.5 for 8 and 48.9 for 16 -- wider windows read status
+//     words past the inclusive one, and that traffic competes with the
+//     tuples'),
 //   - tuples are reordered by digit in LDS and written out, so consecutive
 //     lanes write consecutive addresses of each digit's run.
 // Passes whose digit is constant over the scan (e.g. the top byte when K<29)
@@ -32,16 +39,22 @@
 // every sample's tuples in (k-mer, adj) order, samples one after another, the
 // same as sorting each sample alone.
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "mp_internal.h"
+#include "mp_kmer.h"
 
 namespace mp {
 
-constexpr int kRadix = 256;
-constexpr int kPasses = 10;  // adj, k-mer bytes 0..7, sample of the label
+constexpr int kRadix = kSortRadix;
+constexpr int kPasses = kSortPasses;  // adj, k-mer bytes 0..7, sample of the label
 
 constexpr uint64_t kStAgg = 1, kStInc = 2;
+#ifndef MP_LOOKBACK
+#define MP_LOOKBACK 4
+#endif
+constexpr int kLookBack = MP_LOOKBACK;  // predecessor tiles read per look-back step
 constexpr uint64_t kStVal = (1ull << 46) - 1;
 
 __device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint64_t v)
@@ -106,11 +119,39 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
     return shift < 0 ? (uint32_t)a : (uint32_t)((k >> shift) & 0xFF);
 }
 
-template <int kSB, int kSI>
-__global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
+// The generating pass: its items are the reads' k-mer windows, tile t
+// covering windows [t * tile, (t + 1) * tile) in read order; a window that
+// holds an 'N' or lies outside the scan's buckets is not an item.  The
+// tuples come out in the same order as an enumeration followed by this pass.
+struct GenPass {
+    KParams G;
+    const uint64_t *tile_r0;  // read holding each tile's first window
+    uint64_t tiles;
+    uint64_t npos;            // windows of all reads
+};
+constexpr int kGenReads = 1024;  // reads of a tile staged in LDS (more: read from memory)
+
+// tile_r0[t] = the read holding window t * tile: last r with pos_off[r] <= t * tile
+__global__ void tile_read_kernel(const uint64_t *__restrict__ pos_off, uint64_t N, uint64_t tile, uint64_t tiles,
+                                 uint64_t *__restrict__ tile_r0)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    const uint64_t q = t * tile;
+    uint64_t lo = 0, hi = N - 1;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) >> 1;
+        if (pos_off[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    tile_r0[t] = lo;
+}
+
+template <int kSB, int kSI, bool GEN>
+__global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P, GenPass A)  // two tiles per CU
 {
     constexpr int kSW = kSB / kWave;
     constexpr int kTile = kSB * kSI;
+    static_assert(!GEN || 2 * kGenReads * 8 + (3 * kGenReads + 1) * 4 <= kTile * 8, "read staging exceeds the tile");
     // one staging buffer (keys, then labels + adj) keeps LDS under half a CU,
     // so two tiles run per CU and one tile's memory phases overlap the other's
     // ranking and look-back
@@ -120,7 +161,7 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     __shared__ uint32_t s_dstart[kRadix];  // digit starts in the tile
     __shared__ uint64_t s_gbase[kRadix];
     __shared__ uint32_t s_wsum[kRadix / kWave];
-    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tile, s_nvalid;
 
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
@@ -136,21 +177,77 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     uint8_t a[kSI];
     uint32_t d[kSI];
     uint32_t rank[kSI];
+    if constexpr (GEN) {
+        // reads [ra, rb] overlap the tile's windows; staged in the (idle) staging buffer
+        const uint64_t ra = A.tile_r0[tile], rb = tile + 1 < A.tiles ? A.tile_r0[tile + 1] : A.G.N - 1;
+        const uint32_t nr = (uint32_t)(rb - ra + 1);
+        const bool staged = nr <= (uint32_t)kGenReads;
+        uint64_t *s_rbase = s_buf, *s_rso = s_buf + kGenReads;
+        uint32_t *s_rpos = (uint32_t *)(s_buf + 2 * kGenReads), *s_rlen = s_rpos + kGenReads + 1,
+                 *s_rlab = s_rlen + kGenReads;
+        const uint64_t pa = A.G.pos_off[ra];
+        if (staged) {
+            for (uint32_t j = tid; j <= nr; j += kSB) {
+                s_rpos[j] = (uint32_t)(A.G.pos_off[ra + j] - pa);
+                if (j < nr) {
+                    s_rbase[j] = A.G.base_off[ra + j];
+                    s_rso[j] = A.G.seq_off[ra + j];
+                    s_rlen[j] = A.G.seq_len[ra + j];
+                    s_rlab[j] = label_of(A.G, ra + j);
+                }
+            }
+        }
+        __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kSI; i++) {
-        const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
-        if (idx < P.n) {
-            k[i] = P.keys_in[idx];
-            l[i] = P.lab_in[idx];
-            a[i] = P.adj_in[idx];
-            d[i] = P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a[i], P.shift);
-        } else {
+        for (int i = 0; i < kSI; i++) {
+            const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+            d[i] = kRadix;  // not an item
             k[i] = 0;
             l[i] = 0;
             a[i] = 0;
-            d[i] = kRadix;  // invalid
+            if (idx >= A.npos) continue;
+            const uint32_t q = (uint32_t)(idx - pa);
+            uint32_t lo = 0, hi = nr - 1;  // the read holding window idx: last j with pos[j] <= q
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                const uint32_t pm = staged ? s_rpos[mid] : (uint32_t)(A.G.pos_off[ra + mid] - pa);
+                if (pm <= q) lo = mid; else hi = mid - 1;
+            }
+            const uint64_t r = ra + lo;
+            const uint32_t p = q - (staged ? s_rpos[lo] : (uint32_t)(A.G.pos_off[r] - pa));
+            const uint64_t rbase = staged ? s_rbase[lo] : A.G.base_off[r];
+            uint64_t canon;
+            bool fwd;
+            if (!kmer_at(A.G, rbase + p, p, canon, fwd)) continue;
+            const uint64_t bk = canon >> A.G.bshift;
+            if (bk < A.G.lo || bk >= A.G.hi) continue;
+            k[i] = canon;
+            a[i] = adj_at(A.G, staged ? s_rso[lo] : A.G.seq_off[r], staged ? s_rlen[lo] : A.G.seq_len[r], p, fwd);
+            l[i] = staged ? s_rlab[lo] : label_of(A.G, r);
+            d[i] = digit_of(k[i], a[i], P.shift);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kSI; i++) {
+            const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+            if (idx < P.n) {
+                k[i] = P.keys_in[idx];
+                l[i] = P.lab_in[idx];
+                a[i] = P.adj_in[idx];
+                d[i] = P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a[i], P.shift);
+            } else {
+                k[i] = 0;
+                l[i] = 0;
+                a[i] = 0;
+                d[i] = kRadix;  // invalid
+            }
         }
     }
+    uint32_t ap[(kSI + 3) / 4];  // adj bytes, four per register
+#pragma unroll
+    for (int i = 0; i < (kSI + 3) / 4; i++) ap[i] = 0;
+#pragma unroll
+    for (int i = 0; i < kSI; i++) ap[i / 4] |= (uint32_t)a[i] << (8 * (i % 4));
     // stable rank within the wave: item-major, lane-minor order
     const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -197,41 +294,59 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
         s_dstart[tid] = inc - total;  // wave-local exclusive prefix; waves added below
     }
     __syncthreads();
+    uint32_t ds = 0;
     if (tid < kRadix) {
         uint32_t woff = 0;
         for (int w = 0; w < wave; w++) woff += s_wsum[w];
-        const uint32_t ds = s_dstart[tid] + woff;
+        ds = s_dstart[tid] + woff;
         s_dstart[tid] = ds;
+        if (tid == kRadix - 1) s_nvalid = ds + total;  // items of the tile
 #pragma unroll
         for (int w = 0; w < kSW; w++) s_cnt[w][tid] = (uint16_t)(s_cnt[w][tid] + ds);  // tile position of wave w's first
-        // decoupled look-back for digit tid
-        uint64_t prefix = 0;
-        if (tile != 0) {
-            int64_t j = (int64_t)tile - 1;
-            while (j >= 0) {
-                const uint64_t s = ld_agent(P.status + (uint64_t)j * kRadix + tid);
-                const uint64_t flag = (s >> 46) & 3;
-                if ((uint32_t)(s >> 48) != P.epoch || flag == 0) continue;
-                prefix += s & kStVal;
-                if (flag == kStInc) break;
-                j--;
-            }
-            st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, kStInc, prefix + total));
-        }
-        s_gbase[tid] = P.digit_base[tid] + prefix - ds;  // output index of tile position 0 of this digit
     }
     __syncthreads();
-    // tile-local position of every item, then two staged write-outs
+    // tile-local position of every item, keys staged by it (the items' keys
+    // and digits are dead afterwards, which leaves registers for the look-back)
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
         if (d[i] < (uint32_t)kRadix) {
             rank[i] += s_cnt[wave][d[i]];
             s_buf[rank[i]] = k[i];
             s_dig[rank[i]] = (uint8_t)d[i];
+        } else {
+            rank[i] = ~0u;
         }
     }
+    if (tid < kRadix) {
+        // decoupled look-back for digit tid
+        uint64_t prefix = 0;
+        if (tile != 0) {
+            // kLookBack predecessors per step, their loads in flight together;
+            // tile 0 always publishes an inclusive value, so the walk ends there at the latest
+            int64_t j = (int64_t)tile - 1;
+            bool done = false;
+            while (!done) {
+                uint64_t w[kLookBack];
+#pragma unroll
+                for (int q = 0; q < kLookBack; q++)
+                    w[q] = j - q >= 0 ? ld_agent(P.status + (uint64_t)(j - q) * kRadix + tid) : 0;
+#pragma unroll
+                for (int q = 0; q < kLookBack; q++) {
+                    if (done || j - q < 0) break;
+                    uint64_t s = w[q];
+                    while ((uint32_t)(s >> 48) != P.epoch || ((s >> 46) & 3) == 0)  // not published yet
+                        s = ld_agent(P.status + (uint64_t)(j - q) * kRadix + tid);
+                    prefix += s & kStVal;
+                    done = ((s >> 46) & 3) == kStInc;
+                }
+                j -= kLookBack;
+            }
+            st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, kStInc, prefix + total));
+        }
+        s_gbase[tid] = P.digit_base[tid] + prefix - ds;  // output index of tile position 0 of this digit
+    }
     __syncthreads();
-    const uint32_t nvalid = (uint32_t)(P.n - base < (uint64_t)kTile ? P.n - base : (uint64_t)kTile);
+    const uint32_t nvalid = s_nvalid;
     for (uint32_t j = tid; j < nvalid; j += kSB) {
         P.keys_out[s_gbase[s_dig[j]] + j] = s_buf[j];
     }
@@ -240,9 +355,9 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     uint8_t *s_adj = (uint8_t *)(s_lab + kTile);
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
-        if (d[i] < (uint32_t)kRadix) {
+        if (rank[i] != ~0u) {
             s_lab[rank[i]] = l[i];
-            s_adj[rank[i]] = a[i];
+            s_adj[rank[i]] = (uint8_t)(ap[i / 4] >> (8 * (i % 4)));
         }
     }
     __syncthreads();
@@ -253,19 +368,34 @@ __global__ __launch_bounds__(kSB) void onesweep_kernel(SortPass P)
     }
 }
 
+template <bool GEN>
+static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, const SortPass &P, const GenPass &A)
+{
+    if (SB == 512 && SI == 8)
+        hipLaunchKernelGGL((onesweep_kernel<512, 8, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
+    else if (SB == 512 && SI == 12)
+        hipLaunchKernelGGL((onesweep_kernel<512, 12, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
+    else if (SB == 1024 && SI == 8)
+        hipLaunchKernelGGL((onesweep_kernel<1024, 8, GEN>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P, A);
+    else
+        hipLaunchKernelGGL((onesweep_kernel<1024, 6, GEN>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P, A);
+    MP_HIP(hipGetLastError());
+    return 0;
+}
+
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base)
+                       uint64_t *grp_base, const GenSource *gen)
 {
     *passes_out = 0;
-    if (n <= 1) return 0;
+    if (gen ? gen->npos == 0 : n <= 1) return 0;
     // histograms of every digit
     MP_TRY(ensure(ctx, ctx->hist, sizeof(uint64_t) * (kPasses * kRadix + 64)));
     unsigned long long *gh = (unsigned long long *)ctx->hist.p;
     uint32_t *tile_ctr = (uint32_t *)(gh + kPasses * kRadix);
     MP_TRY(fill_u32(ctx, gh, 0, 2 * kPasses * kRadix + 2 * 32));
     hipEventRecord(ctx->pev[0], ctx->stream);
-    {
+    if (!gen) {
         uint64_t blocks = (n + kRadix * 16 - 1) / (kRadix * 16);
         if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)blocks), dim3(kRadix), 0, ctx->stream,
@@ -275,8 +405,12 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     }
     hipEventRecord(ctx->pev[1], ctx->stream);
     uint64_t h[kPasses * kRadix];
-    MP_HIP(hipMemcpyAsync(h, gh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (gen) {
+        memcpy(h, gen->h, sizeof(h));
+    } else {
+        MP_HIP(hipMemcpyAsync(h, gh, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+    }
     if (!d_grp_first) h[9 * kRadix] = n;  // no samples: one constant digit
     // digit base offsets of the passes that do work
     uint64_t base[kPasses][kRadix];
@@ -288,10 +422,19 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
             acc += h[p * kRadix + r];
             if (h[p * kRadix + r] > mx) mx = h[p * kRadix + r];
         }
-        if (acc != n)
+        if (((pass_mask >> p) & 1) && acc != n)
             return ctx->fail(MP_EDEVICE, "sort histogram total %llu != %llu", (unsigned long long)acc,
                              (unsigned long long)n);
         run[p] = ((pass_mask >> p) & 1) && mx != n;
+    }
+    // the generating pass: the first pass that does work, else the lowest of the mask
+    int gen_p = -1;
+    if (gen) {
+        for (int p = 0; p < kPasses && gen_p < 0; p++)
+            if (run[p]) gen_p = p;
+        if (gen_p < 0) gen_p = __builtin_ctz(pass_mask);
+        if (gen_p < 1 || gen_p > 8) return ctx->fail(MP_EINVAL, "generated sorts take k-mer byte passes only");
+        run[gen_p] = true;
     }
     if (n_grp > 0 && grp_base) {  // tuple offset of every sample after the sort
         for (int g = 0; g < n_grp; g++) grp_base[g] = base[9][g];
@@ -311,15 +454,27 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     }
     const uint64_t kTile = (uint64_t)SB * SI;
     const uint64_t tiles = (n + kTile - 1) / kTile;
-    const uint64_t st_bytes = tiles * kRadix * sizeof(uint64_t);
+    const uint64_t gtiles = gen ? (gen->npos + kTile - 1) / kTile : 0;
+    const uint64_t st_bytes = (tiles > gtiles ? tiles : gtiles) * kRadix * sizeof(uint64_t);
     if (ctx->status.bytes < st_bytes) {
         MP_TRY(ensure(ctx, ctx->status, st_bytes));
         MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
         ctx->epoch = 0;
     }
-    uint64_t *d_base = nullptr;
+    uint64_t *d_base = nullptr, *d_tile_r0 = nullptr;
     MP_HIP(hipMallocAsync((void **)&d_base, sizeof(base), ctx->stream));
     MP_HIP(hipMemcpyAsync(d_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
+    GenPass A{};
+    if (gen) {
+        MP_HIP(hipMallocAsync((void **)&d_tile_r0, gtiles * sizeof(uint64_t), ctx->stream));
+        hipLaunchKernelGGL(tile_read_kernel, dim3((unsigned)((gtiles + 255) / 256)), dim3(256), 0, ctx->stream,
+                           gen->P.pos_off, gen->P.N, kTile, gtiles, d_tile_r0);
+        MP_HIP(hipGetLastError());
+        A.G = gen->P;
+        A.tile_r0 = d_tile_r0;
+        A.tiles = gtiles;
+        A.npos = gen->npos;
+    }
     int executed = 0;
     for (int p = 0; p < kPasses; p++) {
         if (!run[p]) continue;
@@ -344,27 +499,26 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.tile_ctr = tile_ctr + executed;
         P.epoch = ctx->epoch;
         hipEventRecord(ctx->pev[2 + 2 * executed], ctx->stream);
-        if (SB == 512 && SI == 8)
-            hipLaunchKernelGGL((onesweep_kernel<512, 8>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P);
-        else if (SB == 512 && SI == 12)
-            hipLaunchKernelGGL((onesweep_kernel<512, 12>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P);
-        else if (SB == 1024 && SI == 8)
-            hipLaunchKernelGGL((onesweep_kernel<1024, 8>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P);
+        if (p == gen_p)
+            MP_TRY(launch_onesweep<true>(ctx, SB, SI, gtiles, P, A));
         else
-            hipLaunchKernelGGL((onesweep_kernel<1024, 6>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P);
-        MP_HIP(hipGetLastError());
+            MP_TRY(launch_onesweep<false>(ctx, SB, SI, tiles, P, A));
         hipEventRecord(ctx->pev[3 + 2 * executed], ctx->stream);
         *cur = dst;
         executed++;
     }
     MP_HIP(hipFreeAsync(d_base, ctx->stream));
+    if (d_tile_r0) MP_HIP(hipFreeAsync(d_tile_r0, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
     float tot = 0, ms = 0;
+    ctx->sort_gen_ms = 0;
     for (int e = 0; e < executed; e++) {
         hipEventElapsedTime(&ms, ctx->pev[2 + 2 * e], ctx->pev[3 + 2 * e]);
-        tot += ms;
+        if (gen && e == 0) ctx->sort_gen_ms = ms;  // the generating pass, timed apart
+        else tot += ms;
     }
-    ctx->sort_pass_avg = executed ? tot / executed : 0.f;
+    const int plain = executed - (gen && executed ? 1 : 0);
+    ctx->sort_pass_avg = plain ? tot / plain : 0.f;
     hipEventElapsedTime(&ctx->sort_hist_ms, ctx->pev[0], ctx->pev[1]);
     *passes_out = executed;
     return 0;
@@ -380,7 +534,10 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
 // A workgroup owns the groups that start in its 2048-tuple window (staged in
 // LDS with up to 2048 more for the last group); a longer group goes to a list
 // and is sorted by its low bytes with the global passes on its own range.
-constexpr int kLWin = 2048, kLMax = 2048, kLCap = kLWin + kLMax, kLThreads = 256;
+#ifndef MP_LOCAL_WIN
+#define MP_LOCAL_WIN 1024
+#endif
+constexpr int kLWin = MP_LOCAL_WIN, kLMax = MP_LOCAL_WIN, kLCap = kLWin + kLMax, kLThreads = 256;
 
 struct LocalPass {
     const uint64_t *keys_in;
@@ -528,18 +685,25 @@ static int local_bits(int K)
     return ((bits - 30) / 8) * 8;
 }
 
-int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
+static int sort_local_bits(int K) { return getenv("MP_SORT_GLOBAL") ? 0 : local_bits(K); }
+
+uint32_t scan_pass_mask(int K)
+{
+    const int S = sort_local_bits(K);
+    uint32_t mask = 0;
+    for (int b = S / 8; b < 8; b++) mask |= 1u << (b + 1);
+    return mask;
+}
+
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen)
 {
     // k-mer bytes only: within a run nothing downstream depends on the adj
     // order except the quirk's first tuples and the tuple dump, which derive it
-    const int S = getenv("MP_SORT_GLOBAL") ? 0 : local_bits(ctx->cfg.k);
-    if (S == 0 || n <= 1)
-        return radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 0x1FE, passes_out, nullptr, 0,
-                                  nullptr);
-    uint32_t mask = 0;
-    for (int b = S / 8; b < 8; b++) mask |= 1u << (b + 1);
+    const int S = sort_local_bits(ctx->cfg.k);
+    const uint32_t mask = scan_pass_mask(ctx->cfg.k);
     MP_TRY(radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, mask, passes_out, nullptr, 0,
-                              nullptr));
+                              nullptr, gen));
+    if (S == 0 || n <= 1) return 0;
     const uint64_t wins = (n + kLWin - 1) / kLWin;
     MP_TRY(ensure(ctx, ctx->scratch, (wins + 2) * sizeof(uint64_t) + 64));
     unsigned long long *novf = (unsigned long long *)ctx->scratch.p;
@@ -580,7 +744,8 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
         MP_HIP(hipStreamSynchronize(ctx->stream));
         uint32_t lmask = 0;
         for (int b = 0; b < S / 8; b++) lmask |= 1u << (b + 1);
-        const float keep_avg = ctx->sort_pass_avg, keep_hist = ctx->sort_hist_ms;  // stats of the main passes
+        const float keep_avg = ctx->sort_pass_avg, keep_hist = ctx->sort_hist_ms,
+                    keep_gen = ctx->sort_gen_ms;  // stats of the main passes
         for (uint64_t g = 0; g < nl; g++) {
             const uint64_t a = se[g], e = se[nl + g];
             DevArray kv[2], lv[2], av[2];
@@ -603,6 +768,7 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out)
         MP_HIP(hipStreamSynchronize(ctx->stream));
         ctx->sort_pass_avg = keep_avg;
         ctx->sort_hist_ms = keep_hist;
+        ctx->sort_gen_ms = keep_gen;
     }
     ctx->cur = dst;
     *passes_out += 1;

@@ -20,6 +20,10 @@ CASES = [
     ("min3_k27", dict(n_reads=8000, seed=45), 4, 27, 1, 1, True, False, 0),
     ("il_k31", dict(n_reads=8000, seed=46, read_len=150, interleaved=1), 4, 31, 1, 1, False, True, 0),
     ("varlen_k25_T3", dict(n_reads=9000, seed=47, read_len_min=40, read_len_max=180), 6, 25, 1, 3, False, False, 0),
+    # reads of 22-30 bases at K = 27: most have no k-mer or a few, so a sort
+    # tile's windows span thousands of reads (more than its LDS read table)
+    ("short_k27", dict(n_reads=30000, seed=48, read_len_min=22, read_len_max=30, n_rate=0.002), 4, 27, 1, 1, False,
+     False, 0),
     # 40 small genomes: components of a few hundred reads that the pair step
     # chains past 1000 reads, so part of it runs in the reference's order
     ("pairs_ordered", dict(n_reads=20000, n_genomes=40, genome_len_min=4000, genome_len_max=8000, abund_sigma=1.0,
