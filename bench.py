@@ -174,7 +174,10 @@ def main():
     pass_ms = float(sc["ms_sort_pass_avg"])
     per_pass_bytes = 2 * TUPLE_BYTES * n  # read and write 13 B per tuple
     achieved = per_pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
-    sort_bytes = n * (8 + 2 * passes * TUPLE_BYTES)
+    if float(sc["ms_sort_gen"]) > 0:  # the first pass generates the tuples: it writes them and reads the reads
+        sort_bytes = n * TUPLE_BYTES + n * 2 * (passes - 1) * TUPLE_BYTES
+    else:  # enumerated tuples: one histogram read (k-mers), then every pass reads and writes them
+        sort_bytes = n * (8 + 2 * passes * TUPLE_BYTES)
     sort_ms = float(sc["ms_sort"])
     traffic = measured_traffic(args.reads)
     out = {
@@ -201,7 +204,8 @@ def main():
         "sort": {"passes": passes, "digit_bits": 8, "tuple_bytes": TUPLE_BYTES, "algorithmic_bytes": sort_bytes,
                  "ms": sort_ms, "hbm_frac": sort_bytes / (sort_ms * 1e-3) / 1e12 / (HBM_PEAK_GBS / 1e3)
                  if sort_ms > 0 else 0.0},
-        "stages_ms": {k: sc[k] for k in ("ms_enumerate", "ms_sort", "ms_filter", "ms_union")},
+        "stages_ms": {k: sc[k] for k in ("ms_enumerate", "ms_sort", "ms_sort_gen", "ms_sort_local", "ms_filter",
+                                         "ms_union")},
         "finish": plain["finish"],
         "host_stage_s": plain["host_stage_s"],
         "cpu_baseline": cpu,

@@ -198,6 +198,8 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
             }
         }
         __syncthreads();
+        auto rpos = [&](uint32_t j) -> uint32_t { return staged ? s_rpos[j] : (uint32_t)(A.G.pos_off[ra + j] - pa); };
+        uint32_t lo = 0;  // read of the lane's last item: its items are 64 windows apart, so a few steps on
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
             const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
@@ -207,11 +209,15 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
             a[i] = 0;
             if (idx >= A.npos) continue;
             const uint32_t q = (uint32_t)(idx - pa);
-            uint32_t lo = 0, hi = nr - 1;  // the read holding window idx: last j with pos[j] <= q
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                const uint32_t pm = staged ? s_rpos[mid] : (uint32_t)(A.G.pos_off[ra + mid] - pa);
-                if (pm <= q) lo = mid; else hi = mid - 1;
+            // the read holding window idx: last j with pos[j] <= q
+            for (int st = 0; st < 3 && lo + 1 < nr && rpos(lo + 1) <= q; st++) lo++;
+            if (lo + 1 < nr && rpos(lo + 1) <= q) {
+                uint32_t hi = nr - 1;
+                lo++;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (rpos(mid) <= q) lo = mid; else hi = mid - 1;
+                }
             }
             const uint64_t r = ra + lo;
             const uint32_t p = q - (staged ? s_rpos[lo] : (uint32_t)(A.G.pos_off[r] - pa));
