@@ -652,6 +652,8 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
     const uint64_t rwords = (n + 63) / 64 + 1;
     MP_TRY(ensure(ctx, ctx->runbits, 2 * rwords * sizeof(uint64_t)));
     uint64_t *rs_bits = (uint64_t *)ctx->runbits.p, *st_bits = rs_bits + rwords;
+    if (!grid_fits((n + kFTile - 1) / kFTile, kFLanes))
+        return ctx->fail(MP_EINVAL, "filter over %llu tuples exceeds one dispatch: use more scans", (unsigned long long)n);
     if (n) {
         hipLaunchKernelGGL(filter_tile_kernel, dim3((unsigned)((n + kFTile - 1) / kFTile)), dim3(kFLanes), 0,
                            ctx->stream, G, rs_bits, st_bits);

@@ -58,6 +58,15 @@ inline unsigned stride_grid(uint64_t n)
     const uint64_t b = (n + 255) / 256;
     return (unsigned)(b < 2048 ? (b ? b : 1) : 2048);
 }
+// workgroups of 256 lanes for a wave-per-item grid-stride loop over n items:
+// one wave per item up to 2^20 workgroups (a dispatch holds < 2^32 lanes)
+inline unsigned wave_grid(uint64_t n)
+{
+    const uint64_t b = (n + 3) / 4;
+    return (unsigned)(b < (1u << 20) ? (b ? b : 1) : (1u << 20));
+}
+// a dispatch holds fewer than 2^32 work-items (the packet's grid size is 32-bit)
+inline bool grid_fits(uint64_t blocks, uint64_t lanes) { return blocks * lanes < (1ull << 32); }
 
 // Reverse-complement of a 2-bit packed k-mer (first base most significant,
 // A0 C1 G2 T3, complement = 3 - code): reverse the 2-bit groups of ~x.
@@ -175,6 +184,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
 // per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
 int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
+// 3-minimizer k-mers of a scan: ordered enumeration into the tuple buffer 0
 int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
 // The scan's tuples generated by the first radix pass instead of an
 // enumeration: the reads' k-mers in [lo, hi) and their digit histograms.

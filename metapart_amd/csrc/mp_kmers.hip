@@ -1,10 +1,11 @@
-// Canonical k-mer kernels: m-mer histograms and ordered tuple enumeration.
+// Canonical k-mer kernels: m-mer histograms, digit counts and the ordered
+// 3-minimizer enumeration.
 //
 // Plain variant (every k-mer; countMmers.c:83-217, enumerateKmers.c:127-371).
 // The reference rolls a forward and a reverse-complement code through each
-// read on one CPU thread.  Here a workgroup owns 64 consecutive reads and its
-// 256 lanes walk the flattened list of their k-mer positions, 256 at a time,
-// so every lane does O(1) work per k-mer regardless of read length:
+// read on one CPU thread.  Here the lanes of a workgroup walk the flattened
+// list of their reads' k-mer positions, so every lane does O(1) work per
+// k-mer regardless of read length (kmer_at):
 //   F  = 2K bits of the packed stream at the k-mer's first base (two 64-bit
 //        loads and a funnel shift; MSB-first packing makes this the
 //        reference's curKmerInt),
@@ -13,13 +14,17 @@
 //   canonical = F < R ? F : R, bucket = canonical >> 2(K-10) (getMmerPrefix).
 // At K = 32 the reference's mask (1 << 64) - 1 evaluates to 0 on x86, so every
 // forward code after the first position is 0; that is reproduced.
-// Emission is an ordered stream compaction (wave ballots + per-workgroup
-// offsets from a count pass), so the tuple array is in the reference's
-// generation order: read order, then position order within a read.
+// The plain tuples themselves are not enumerated into memory: the first radix
+// pass of the sort generates them from the reads in the same order (read
+// order, then position order within a read).  This file gives it the digit
+// counts: the m-mer histogram's counting pass also counts the k-mer bytes the
+// sort reads, and kmer_digit_hist_kernel counts a scan that holds part of the
+// buckets.
 //
 // 3-minimizer variant (countMmers_3minimizers.c:83-271,
 // enumerateKmers_3minimizers.c:129-474): a sequential window state machine per
-// read, one lane per read, over the same O(1) canonical k-mers.
+// read, one lane per read, over the same O(1) canonical k-mers; its tuples are
+// enumerated (ordered stream compaction with offsets from a count pass).
 #include "mp_internal.h"
 #include "mp_kmer.h"
 
@@ -28,101 +33,6 @@
 #include <vector>
 
 namespace mp {
-
-template <int MODE>
-__global__ __launch_bounds__(kKBlock) void kmer_plain_kernel(KParams P)
-{
-    __shared__ uint32_t s_pref[kRG + 1];
-    __shared__ uint64_t s_base[kRG], s_so[kRG];
-    __shared__ uint32_t s_len[kRG], s_aux[kRG];
-    __shared__ uint32_t s_wc[kKBlock / kWave];
-    const uint64_t r0 = (uint64_t)blockIdx.x * kRG;
-    const int nr = (int)((P.N - r0) < (uint64_t)kRG ? (P.N - r0) : (uint64_t)kRG);
-    const int tid = threadIdx.x;
-    const uint64_t p0 = P.pos_off[r0];
-    if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
-    if (tid < nr) {
-        const uint64_t r = r0 + tid;
-        s_base[tid] = P.base_off[r];
-        s_so[tid] = P.seq_off[r];
-        s_len[tid] = P.seq_len[r];
-        if (MODE == kModeHist) {
-            int lo = 0, hi = P.C - 1;  // chunk: last c with chunk_ids[c] <= r
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (P.chunk_ids[mid] <= r) lo = mid; else hi = mid - 1;
-            }
-            s_aux[tid] = (uint32_t)lo;
-        } else if (MODE == kModeEmit) {
-            s_aux[tid] = P.scan0 ? (uint32_t)r : (P.bad[r] ? (uint32_t)r : P.parent[r]);
-        }
-    }
-    __syncthreads();
-    const uint32_t total = s_pref[nr];
-    uint32_t count = 0;
-    uint64_t out = MODE == kModeEmit ? P.wg_off[blockIdx.x] : 0;
-    const int wave = tid / kWave;
-    for (uint32_t q0 = 0; q0 < total; q0 += kKBlock) {
-        const uint32_t q = q0 + tid;
-        bool v = false, fwd = false;
-        uint64_t canon = 0;
-        int rl = 0;
-        uint32_t p = 0;
-        if (q < total) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_pref[mid] <= q) lo = mid; else hi = mid - 1;
-            }
-            rl = lo;
-            p = q - s_pref[rl];
-            v = kmer_at(P, s_base[rl] + p, p, canon, fwd);
-            if (MODE != kModeHist && v) {
-                const uint64_t bk = canon >> P.bshift;
-                v = bk >= P.lo && bk < P.hi;
-            }
-        }
-        if (MODE == kModeHist) {
-            if (v) atomicAdd(&P.counts[(size_t)s_aux[rl] * (MP_NBUCKETS + 2) + (canon >> P.bshift)], 1u);
-        } else if (MODE == kModeCount) {
-            count += v;
-        } else {
-            const uint64_t m = __ballot(v);
-            if (__lane_id() == 0) s_wc[wave] = (uint32_t)__popcll(m);
-            __syncthreads();
-            uint32_t woff = 0, tot = 0;
-#pragma unroll
-            for (int w = 0; w < kKBlock / kWave; w++) {
-                woff += w < wave ? s_wc[w] : 0;
-                tot += s_wc[w];
-            }
-            if (v) {
-                const uint64_t pos = out + woff + (uint64_t)__popcll(m & lanemask_lt());
-                const uint64_t so = s_so[rl];
-                const uint32_t L = s_len[rl];
-                const int K = P.K;
-                const uint8_t prev = p == 0 ? (uint8_t)'N' : P.fq[so + p - 1];
-                const uint8_t next = p == L - K ? (uint8_t)'N' : P.fq[so + p + K];
-                const uint8_t a = fwd ? adj_of(prev, next) : adj_of(rc_char(next), rc_char(prev));
-                P.keys[pos] = canon;
-                P.adj[pos] = a;
-                P.lab[pos] = s_aux[rl];
-            }
-            out += tot;
-            __syncthreads();
-        }
-    }
-    if (MODE == kModeCount) {
-        for (int d = kWave / 2; d > 0; d >>= 1) count += __shfl_down(count, d, kWave);
-        if (__lane_id() == 0) s_wc[wave] = count;
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t t = 0;
-            for (int w = 0; w < kKBlock / kWave; w++) t += s_wc[w];
-            P.wg_cnt[blockIdx.x] = t;
-        }
-    }
-}
 
 // ---------------------------------------------------------------- m-mer histogram
 // The per-chunk table has 2^20 bins per chunk, far beyond LDS, and one global
@@ -644,21 +554,19 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
 
 int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out)
 {
+    if (!ctx->cfg.min3) return ctx->fail(MP_EINVAL, "k-mer tuples of every window come from the generating sort pass");
     KParams P = base_params(ctx);
     P.lo = lo;
     P.hi = hi;
     P.scan0 = s == 0;
     P.parent = (const uint32_t *)ctx->parent.p;
     P.bad = (const uint8_t *)ctx->bad.p;
-    const bool m3 = ctx->cfg.min3 != 0;
-    const uint64_t per = m3 ? kKBlock : kRG;
-    const uint64_t nwg = (ctx->N + per - 1) / per;
+    const uint64_t nwg = (ctx->N + kKBlock - 1) / kKBlock;
     MP_TRY(ensure(ctx, ctx->wg_cnt, nwg * sizeof(uint32_t)));
     MP_TRY(ensure(ctx, ctx->wg_off, (nwg + 1) * sizeof(uint64_t)));
     P.wg_cnt = (uint32_t *)ctx->wg_cnt.p;
     P.wg_off = (const uint64_t *)ctx->wg_off.p;
-    if (m3) hipLaunchKernelGGL(kmer_min3_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
-    else hipLaunchKernelGGL(kmer_plain_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
+    hipLaunchKernelGGL(kmer_min3_kernel<kModeCount>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
     MP_HIP(hipGetLastError());
     MP_TRY(scan_u32(ctx, (const uint32_t *)ctx->wg_cnt.p, (uint64_t *)ctx->wg_off.p, nwg));
     uint64_t n = 0;
@@ -673,8 +581,7 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
     P.adj = (uint8_t *)ctx->adj[0].p;
     P.lab = (uint32_t *)ctx->lab[0].p;
     if (n > 0) {
-        if (m3) hipLaunchKernelGGL(kmer_min3_kernel<kModeEmit>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
-        else hipLaunchKernelGGL(kmer_plain_kernel<kModeEmit>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
+        hipLaunchKernelGGL(kmer_min3_kernel<kModeEmit>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
         MP_HIP(hipGetLastError());
     }
     ctx->cur = 0;

@@ -149,10 +149,11 @@ __global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restr
                                                           const uint64_t *__restrict__ base_off, uint64_t n_rec,
                                                           uint8_t *__restrict__ dense)
 {
-    const uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    if (r >= n_rec) return;
-    const uint64_t src = seq_off[r], dst = base_off[r], len = base_off[r + 1] - dst;
-    for (uint64_t b = __lane_id(); b < len; b += kWave) dense[dst + b] = fq[src + b];
+    const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n_rec; r += waves) {
+        const uint64_t src = seq_off[r], dst = base_off[r], len = base_off[r + 1] - dst;
+        for (uint64_t b = __lane_id(); b < len; b += kWave) dense[dst + b] = fq[src + b];
+    }
 }
 
 __global__ __launch_bounds__(256) void pack_dense_kernel(const uint8_t *__restrict__ dense, uint64_t total_bases,
@@ -214,6 +215,8 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     if (nbytes == 0) return ctx->fail(MP_EINVAL, "empty FASTQ");
     if (((uintptr_t)fq) & 15) return ctx->fail(MP_EINVAL, "FASTQ device buffer must be 16-byte aligned");
     const uint64_t nblk = (nbytes + kParseBlockBytes - 1) / kParseBlockBytes;
+    if (!grid_fits(nblk, kParseBlock)) return ctx->fail(MP_EINVAL, "FASTQ of %llu bytes exceeds one dispatch",
+                                                        (unsigned long long)nbytes);
 
     // newline counts and offsets (scratch space: wg_cnt / wg_off)
     MP_TRY(ensure(ctx, ctx->wg_cnt, nblk * sizeof(uint32_t)));
@@ -308,7 +311,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     MP_TRY(ensure(ctx, ctx->keys[1], nthreads * 64 + 64));
     uint8_t *dense = (uint8_t *)ctx->keys[1].p;
     if (N) {
-        hipLaunchKernelGGL(dense_bases_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream, fq,
+        hipLaunchKernelGGL(dense_bases_kernel, dim3(wave_grid(N)), dim3(256), 0, ctx->stream, fq,
                            (const uint64_t *)ctx->seq_off.p, (const uint64_t *)ctx->base_off.p, N, dense);
         MP_HIP(hipGetLastError());
     }

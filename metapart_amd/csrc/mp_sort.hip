@@ -377,6 +377,9 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
 template <bool GEN>
 static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, const SortPass &P, const GenPass &A)
 {
+    if (!grid_fits(tiles, SB))
+        return ctx->fail(MP_EINVAL, "radix pass of %llu tiles exceeds one dispatch: use more scans",
+                         (unsigned long long)tiles);
     if (SB == 512 && SI == 8)
         hipLaunchKernelGGL((onesweep_kernel<512, 8, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
     else if (SB == 512 && SI == 12)
@@ -726,6 +729,9 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     L.shift = S;
     L.ovf = (uint64_t *)ctx->scratch.p + 1;
     L.novf = novf;
+    if (!grid_fits(wins, kLThreads))
+        return ctx->fail(MP_EINVAL, "local pass of %llu windows exceeds one dispatch: use more scans",
+                         (unsigned long long)wins);
     hipEventRecord(ctx->pev[22], ctx->stream);
     hipLaunchKernelGGL(local_rank_kernel, dim3((unsigned)wins), dim3(kLThreads), 0, ctx->stream, L);
     MP_HIP(hipGetLastError());

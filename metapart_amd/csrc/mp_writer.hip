@@ -113,19 +113,20 @@ __global__ void gather_len_kernel(const uint32_t *__restrict__ slot, const uint3
     if (i < n) out[i] = len[slot[i]];
 }
 
-// one wave per item: lanes copy consecutive bytes
+// one wave per item (grid-stride over items): lanes copy consecutive bytes
 __global__ void copy_items_kernel(const uint8_t *__restrict__ fq, const uint64_t *__restrict__ rec_off,
                                   const uint32_t *__restrict__ slot, const uint64_t *__restrict__ off, uint64_t n_items,
                                   uint8_t *__restrict__ out)
 {
-    const uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    if (item >= n_items) return;
     const unsigned lane = __lane_id();
-    const uint64_t r = slot[item] >> 1;
-    const uint64_t src = rec_off[r];
-    const uint64_t dst = off[item];
-    const uint64_t len = off[item + 1] - dst;
-    for (uint64_t b = lane; b < len; b += kWave) out[dst + b] = fq[src + b];
+    const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    for (uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; item < n_items; item += waves) {
+        const uint64_t r = slot[item] >> 1;
+        const uint64_t src = rec_off[r];
+        const uint64_t dst = off[item];
+        const uint64_t len = off[item + 1] - dst;
+        for (uint64_t b = lane; b < len; b += kWave) out[dst + b] = fq[src + b];
+    }
 }
 
 // first item of every file (and the end), and the byte offset there
@@ -246,8 +247,7 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     MP_TRY(ensure(ctx, ctx->out, total + 64));
     const uint64_t real_items = fpos[F];
     if (real_items) {
-        const uint64_t threads = real_items * kWave;
-        hipLaunchKernelGGL(copy_items_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(copy_items_kernel, dim3(wave_grid(real_items)), dim3(256), 0, ctx->stream,
                            ctx->d_fq, (const uint64_t *)ctx->rec_off.p, slot, (const uint64_t *)off, real_items,
                            (uint8_t *)ctx->out.p);
         MP_HIP(hipGetLastError());

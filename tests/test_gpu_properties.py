@@ -105,3 +105,33 @@ def test_scan_labels_are_deterministic_and_compressed():
         assert np.all(par[par[ok]] == par[ok])
         seen.add(hashlib.sha256(par.tobytes()).hexdigest())
     assert len(seen) == 1
+
+
+@pytest.mark.gpu
+def test_wave_per_record_kernels_cover_every_read(tmp_path):
+    # 4.6 M reads: more records than the capped wave-per-record grids hold
+    # (2^20 workgroups x 4 waves), so the packing copy and the writer's record
+    # copy loop over records; the reads past that point must pack and write
+    # like the others
+    print("gpu wave-per-record run: 4.6M reads", flush=True)
+    n, tail = 4_600_000, 50_000
+    buf = mp.synth(0, n_reads=n, read_len=40, n_genomes=50, genome_len_min=20000, genome_len_max=40000, seed=31)
+    v = np.frombuffer(buf.tobytes(), dtype=np.uint8)
+    nl = np.flatnonzero(v == 10)
+    cut = int(nl[4 * (n - tail) - 1]) + 1  # first byte of record n - tail
+    with mp.Context(k=21) as ctx:
+        ctx.load(buf, [0, cut, v.size])
+        _, _, _, full = ctx.count(want_counts=True)
+    with mp.Context(k=21) as ctx:
+        ctx.load(v[cut:].copy(), [0, v.size - cut])
+        _, _, _, part = ctx.count(want_counts=True)
+    assert np.array_equal(full[1], part[0])
+    (tmp_path / "Rank0").mkdir()
+    with mp.Context(k=21) as ctx:
+        ctx.load(buf, mp.index_buffer(buf, 8))
+        r = mp.partition(ctx, S=1, T=1, out_prefix=str(tmp_path))
+    files = sorted((tmp_path / "Rank0").iterdir())
+    out = np.concatenate([np.fromfile(str(f), dtype=np.uint8) for f in files])
+    assert r.out_bytes == out.size == v.size
+    assert int(out.astype(np.uint64).sum()) == int(v.astype(np.uint64).sum())
+    assert int(np.count_nonzero(out == ord("@"))) == int(np.count_nonzero(v == ord("@")))
