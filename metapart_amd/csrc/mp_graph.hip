@@ -188,10 +188,12 @@ __global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_
                 bool zeroed = false;
                 if (e != s1) {  // the last run of a segment is never evaluated
                     ev++;
-                    zeroed = eval_run(
-                                 G, i, e,
-                                 [&](uint64_t t) -> uint32_t { return t - t0 < cnt ? s_adj[t - t0] : G.adj[t]; },
-                                 [&](uint64_t t) -> uint32_t { return t - t0 < cnt ? s_lab[t - t0] : G.lab[t]; }) == 2;
+                    if (e <= t0 + cnt)  // inside the tile: LDS only
+                        zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t { return s_adj[t - t0]; },
+                                          [&](uint64_t t) -> uint32_t { return s_lab[t - t0]; }) == 2;
+                    else  // the run crosses the tile's end (at most one per tile): memory
+                        zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t { return G.adj[t]; },
+                                          [&](uint64_t t) -> uint32_t { return G.lab[t]; }) == 2;
                     z += zeroed;
                 }
                 rs = k != 0 && !zeroed && e - i >= 2;
