@@ -267,44 +267,62 @@ __device__ __forceinline__ bool is_bad(const uint64_t *bits, uint32_t x) { retur
 // One lane per tuple; a lane whose tuple starts a run to join finds the run's
 // end in the run-start bitmap and star-joins the non-BAD labels to the first
 // non-BAD one (metapart.c:2539-2616).
-__global__ void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits, const uint64_t *__restrict__ st_bits,
-                             const uint64_t *__restrict__ badbits)
+// Union of every run to join (metapart.c:2520-2800).  The reference joins a
+// run's live labels to its first one (a star); a chain through the live
+// labels in run order joins the same sets, and lets a lane take one tuple:
+// each wave takes a 64-tuple word of the run bitmaps, loads its labels
+// (coalesced), and every live tuple (in a run to join, label not BAD) joins
+// the nearest live tuple before it in its run -- found by a ballot inside the
+// word, or, for the first live tuple of a run that began in an earlier word,
+// by a walk back through that run.  Most tuples of this workload have BAD
+// labels, so the work is the label and BAD-bit reads, not the finds.
+__global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits,
+                                                    const uint64_t *__restrict__ st_bits,
+                                                    const uint64_t *__restrict__ badbits)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < G.n; i += stride) {
-        if (!((rs_bits[i >> 6] >> (i & 63)) & 1)) continue;
-        uint64_t q = i >> 6;
-        uint64_t m = st_bits[q] & ~((2ull << (i & 63)) - 1);  // run starts after i in its word
-        uint64_t e = G.n;
-        for (;;) {
+    const uint64_t nw = (G.n + 63) / 64;
+    const unsigned lane = __lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
+    const uint64_t lt = lanemask_lt();
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; w < nw; w += nwaves) {
+        const uint64_t st = st_bits[w], rs = rs_bits[w];
+        // the run holding lane 0 when it began before this word: its start and whether it is joined
+        uint64_t cs = 0;
+        bool cjoin = false;
+        if (!(st & 1) && w > 0) {
+            uint64_t q = w - 1, m = st_bits[q];
+            while (!m && q > 0) m = st_bits[--q];
             if (m) {
-                e = min(G.n, (q << 6) + (uint64_t)__builtin_ctzll(m));
-                break;
+                const int bit = 63 - __builtin_clzll(m);
+                cs = q * 64 + bit;
+                cjoin = (rs_bits[q] >> bit) & 1;
             }
-            if (++q << 6 >= G.n) break;
-            m = st_bits[q];
         }
-        uint64_t c = i;
-        while (c < e && is_bad(badbits, G.lab[c])) c++;
-        if (c >= e) continue;
-        const uint32_t a = G.lab[c];
-        // star around a, keeping a's root between members: a stale root is
-        // still a valid hook target (it only grows towards the real root)
-        uint32_t ra = uf_find(G.parent, a);
-        for (uint64_t t = c + 1; t < e; t++) {
-            const uint32_t b = G.lab[t];
-            if (b == a || is_bad(badbits, b)) continue;
-            uint32_t rb = uf_find(G.parent, b);
-            while (ra != rb) {
-                const uint32_t lo = ra < rb ? ra : rb, hi = ra < rb ? rb : ra;
-                uint32_t expect = lo;  // hook the smaller root under the larger (unionfind.h:58-69)
-                if (__hip_atomic_compare_exchange_strong(&G.parent[lo], &expect, hi, __ATOMIC_RELAXED,
-                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    ra = hi;
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+        const uint64_t sm = st & upto;
+        const int sbit = sm ? 63 - __builtin_clzll(sm) : -1;  // start of the lane's run in this word
+        const bool join = sbit >= 0 ? ((rs >> sbit) & 1) : cjoin;
+        const uint64_t t = w * 64 + lane;
+        uint32_t b = 0;
+        bool live = false;
+        if (t < G.n && join) {
+            b = G.lab[t];
+            live = !is_bad(badbits, b);
+        }
+        uint64_t cand = __ballot(live) & lt;
+        if (sbit > 0) cand &= ~((1ull << sbit) - 1);  // lanes of the same run
+        const int p = cand ? 63 - __builtin_clzll(cand) : (int)lane;
+        const uint32_t a = __shfl(b, p, kWave);
+        if (!live) continue;
+        if (cand) {
+            if (a != b) uf_union(G.parent, a, b);
+        } else if (sbit < 0) {  // first live tuple of an earlier run in this word: the run's live tuple before it
+            for (uint64_t u = w * 64; u > cs;) {
+                const uint32_t c = G.lab[--u];
+                if (!is_bad(badbits, c)) {
+                    if (c != b) uf_union(G.parent, c, b);
                     break;
                 }
-                ra = uf_find(G.parent, ra);
-                rb = uf_find(G.parent, rb);
             }
         }
     }
@@ -697,9 +715,8 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
         hipLaunchKernelGGL(bad_bits_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint8_t *)G.bad, N,
                            badbits);
         MP_HIP(hipGetLastError());
-        const uint64_t ub = std::min<uint64_t>(nblk(n), 256 * 16);
-        hipLaunchKernelGGL(union_kernel, dim3((unsigned)ub), dim3(256), 0, ctx->stream, G, (const uint64_t *)rs_bits,
-                           (const uint64_t *)st_bits, (const uint64_t *)badbits);
+        hipLaunchKernelGGL(union_kernel, dim3(wave_grid((n + 63) / 64)), dim3(256), 0, ctx->stream, G,
+                           (const uint64_t *)rs_bits, (const uint64_t *)st_bits, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
     }
     MP_TRY(compress_labels(ctx, G.parent, (const uint8_t *)G.bad, N));
