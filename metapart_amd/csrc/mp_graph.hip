@@ -65,147 +65,247 @@ __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, in
     pos[t] = lo;
 }
 
-// Evaluates run [i, j) (not the last run of its segment): the in/out
-// neighbour classes of its tuples decide whether the k-mer is zeroed (both
-// sides branch) or which labels are filtered.  Returns 1, or 2 when zeroed.
-template <typename AdjAt, typename LabAt>
-__device__ int eval_run(const GParams &G, uint64_t i, uint64_t j, AdjAt adj_at, LabAt lab_at)
+// A run's verdict from its in/out neighbour-class tallies (metapart.c:2300-2359,
+// 2419): 0 = nothing, kVZero = zero the k-mer (both sides branch), else
+// kVMark with the rule each tuple's label is filtered by.
+constexpr uint32_t kVMark = 1, kVZero = 2, kVRatio = 4, kVIn = 8, kVOut = 16;
+
+// One side of metapart.c:2300-2333 over classes 0-6: the number of classes
+// present, the largest count and its class (the first on ties), and the
+// second-largest count (with multiplicity).  Branch-free: the maximum of
+// count << 3 | (7 - class) is the largest count at its lowest class.
+__device__ __forceinline__ void class_stats(const uint32_t cnt[8], int &n, uint32_t &mx, int &arg, uint32_t &second)
+{
+    uint32_t best = 0;
+    n = 0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+        best = max(best, cnt[c] ? (cnt[c] << 3) | (uint32_t)(7 - c) : 0u);
+        n += cnt[c] != 0;
+    }
+    mx = best >> 3;
+    arg = 7 - (int)(best & 7);
+    second = 0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) second = max(second, c == arg ? 0u : cnt[c]);
+}
+
+__device__ __forceinline__ uint32_t run_verdict(const uint32_t in[8], const uint32_t out[8], uint64_t len)
+{
+    int nIn, nOut, maxIn, maxOut;
+    uint32_t mIn, mOut, sIn, sOut;
+    class_stats(in, nIn, mIn, maxIn, sIn);
+    class_stats(out, nOut, mOut, maxOut, sOut);
+    if (!(nIn > 1 || nOut > 1)) return 0;
+    if (nIn > 1 && nOut > 1) return kVZero;
+    // (float)smax / degree compared with the double 0.2 (metapart.c:2356-2359, 2419)
+    const float deg = (float)(int)len;
+    const bool ratio = (double)__fdiv_rn((float)sIn, deg) > 0.2 || (double)__fdiv_rn((float)sOut, deg) > 0.2;
+    return kVMark | (ratio ? kVRatio : 0) | (nIn > 1 ? kVIn : 0) | (nOut > 1 ? kVOut : 0) | ((uint32_t)maxIn << 8) |
+           ((uint32_t)maxOut << 12);
+}
+
+// filteredReads for a tuple of a kVMark run; the refilter step turns it into
+// BAD (the label is a root here, so that step's parent reset changes nothing for it)
+__device__ __forceinline__ bool tuple_filtered(uint32_t v, uint32_t a)
+{
+    return ((v & kVIn) && (a & 7) != ((v >> 8) & 7)) || ((v & kVOut) && ((a >> 3) & 7) != ((v >> 12) & 7)) ||
+           (v & kVRatio);
+}
+
+__device__ void zero_run(const GParams &G, uint64_t i, uint64_t j)
+{
+    for (uint64_t t = i; t < j;) {
+        const uint64_t w = t >> 5;
+        const uint64_t e = ((w + 1) << 5) < j ? ((w + 1) << 5) : j;
+        const uint32_t nb = (uint32_t)(e - t);
+        const uint32_t m = (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1)) << (t & 31);
+        atomicOr(&G.zbits[w], m);
+        t = e;
+    }
+}
+
+// Evaluates run [i, j) (not the last run of its segment) from memory and
+// applies its verdict.  Returns 1, or 2 when zeroed.
+__device__ int eval_run_mem(const GParams &G, uint64_t i, uint64_t j)
 {
     uint32_t in[8], out[8];
-    if (j - i < 65536) {  // tallies in 16-bit fields: classes 0-3 and 4-7 in two words per side
-        uint64_t il = 0, ih = 0, ol = 0, oh = 0;
-        for (uint64_t t = i; t < j; t++) {
-            const uint32_t a = adj_at(t), ci = a & 7, co = (a >> 3) & 7;
-            const uint64_t bi = 1ull << (16 * (ci & 3)), bo = 1ull << (16 * (co & 3));
-            il += ci < 4 ? bi : 0;
-            ih += ci < 4 ? 0 : bi;
-            ol += co < 4 ? bo : 0;
-            oh += co < 4 ? 0 : bo;
-        }
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            in[c] = (uint32_t)(il >> (16 * c)) & 0xFFFF;
-            in[c + 4] = (uint32_t)(ih >> (16 * c)) & 0xFFFF;
-            out[c] = (uint32_t)(ol >> (16 * c)) & 0xFFFF;
-            out[c + 4] = (uint32_t)(oh >> (16 * c)) & 0xFFFF;
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 8; c++) in[c] = out[c] = 0;
-        for (uint64_t t = i; t < j; t++) {
-            const uint32_t a = adj_at(t);
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                in[c] += (a & 7) == (uint32_t)c;
-                out[c] += ((a >> 3) & 7) == (uint32_t)c;
-            }
-        }
+    for (int c = 0; c < 8; c++) in[c] = out[c] = 0;
+    for (uint64_t t = i; t < j; t++) {
+        const uint32_t a = G.adj[t];
+        in[a & 7]++;
+        out[(a >> 3) & 7]++;
     }
-    int nIn = 0, nOut = 0, maxIn = 0, maxOut = 0;
-    uint32_t mIn = 0, mOut = 0, sIn = 0, sOut = 0;
-#pragma unroll
-    for (int c = 0; c < 7; c++) {  // metapart.c:2300-2333, ties keep the first maximum
-        if (in[c]) {
-            if (in[c] > mIn) { sIn = mIn; mIn = in[c]; maxIn = c; }
-            else if (in[c] > sIn) sIn = in[c];
-            nIn++;
-        }
-        if (out[c]) {
-            if (out[c] > mOut) { sOut = mOut; mOut = out[c]; maxOut = c; }
-            else if (out[c] > sOut) sOut = out[c];
-            nOut++;
-        }
-    }
-    if (!(nIn > 1 || nOut > 1)) return 1;
-    if (nIn > 1 && nOut > 1) {  // both sides branch: the k-mer is zeroed
-        for (uint64_t t = i; t < j;) {
-            const uint64_t w = t >> 5;
-            const uint64_t e = ((w + 1) << 5) < j ? ((w + 1) << 5) : j;
-            const uint32_t nb = (uint32_t)(e - t);
-            const uint32_t m = (nb == 32 ? 0xFFFFFFFFu : ((1u << nb) - 1)) << (t & 31);
-            atomicOr(&G.zbits[w], m);
-            t = e;
-        }
+    const uint32_t v = run_verdict(in, out, j - i);
+    if (v & kVZero) {
+        zero_run(G, i, j);
         return 2;
     }
-    // (float)smax / degree compared with the double 0.2 (metapart.c:2356-2359, 2419)
-    const float deg = (float)(int)(j - i);
-    const bool ratio = (double)__fdiv_rn((float)sIn, deg) > 0.2 || (double)__fdiv_rn((float)sOut, deg) > 0.2;
-    for (uint64_t t = i; t < j; t++) {
-        const uint8_t a = adj_at(t);
-        const int bi = a & 7, ai = (a >> 3) & 7;
-        if ((nIn > 1 && bi != maxIn) || (nOut > 1 && ai != maxOut) || ratio) {
-            // filteredReads; the refilter step turns it into BAD (the label is a
-            // root here, so that step's parent reset changes nothing for it)
-            G.filtered[lab_at(t)] = 1;
-        }
-    }
+    if (v & kVMark)
+        for (uint64_t t = i; t < j; t++)
+            if (tuple_filtered(v, G.adj[t])) G.filtered[G.lab[t]] = 1;
     return 1;
 }
 
-// Filter over tiles of 2048 tuples staged in LDS (keys, labels and adj read once).
-// Lane l of the workgroup owns tuples l, l+256, ... of the tile (consecutive
-// lanes read consecutive keys, no bank conflicts) and evaluates the runs that
-// start there, reading past the tile end only for a run that crosses it.
-// Besides the filter's effects it records two bitmaps for the union, one
-// wave ballot per 64 tuples: every run start (segment starts included) and
-// the starts of the runs the union must join (k-mer not zero and not zeroed,
-// two tuples or more).
-constexpr int kFLanes = 256, kFPer = 8, kFTile = kFLanes * kFPer;
+// Filter over tiles of 2048 tuples, looking 256 tuples past the tile so that
+// a run crossing its end is usually whole.  Nothing of the tuples is staged
+// in LDS but bitmaps: lane l of the workgroup owns tuples l, l+256, ... (a
+// wave owns 64 consecutive tuples, a "word").  Three steps:
+//   1. per word, wave ballots give the run starts (segment starts included),
+//      the tuples with a non-zero k-mer and three bit planes each of the in
+//      and out neighbour classes; the run starts of the tile go to a list;
+//   2. one lane per listed run finds the run's end in the start bitmap, its
+//      class tallies by popcounts over the run's words, and its verdict; a run
+//      longer than the tuples looked at is evaluated from memory;
+//   3. every tuple whose run starts in the tile applies the run's verdict to
+//      its label (the tile before owns the runs that start before).
+// Besides the filter's effects it records two bitmaps for the union: every
+// run start and the starts of the runs the union must join (k-mer not zero
+// and not zeroed, two tuples or more).
+constexpr int kFLanes = 256, kFPer = 8, kFTile = kFLanes * kFPer, kFHalo = 256, kFStage = kFTile + kFHalo,
+              kFIter = kFStage / kFLanes, kFWords = kFStage / 64, kFSeg = 64;
 
-__global__ __launch_bounds__(kFLanes) void filter_tile_kernel(GParams G, uint64_t *__restrict__ rs_bits,
-                                                              uint64_t *__restrict__ st_bits)
+__device__ __forceinline__ uint32_t class_count(const uint64_t *pl, uint64_t m, int c)
 {
-    __shared__ uint64_t s_key[kFTile];
-    __shared__ uint32_t s_lab[kFTile];
-    __shared__ uint8_t s_adj[kFTile];
+    m &= (c & 1) ? pl[0] : ~pl[0];
+    m &= (c & 2) ? pl[1] : ~pl[1];
+    m &= (c & 4) ? pl[2] : ~pl[2];
+    return (uint32_t)__popcll(m);
+}
+
+__global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint64_t *__restrict__ rs_bits,
+                                                                 uint64_t *__restrict__ st_bits)
+{
+    __shared__ uint64_t s_st[kFWords], s_nz[kFWords], s_rs[kFWords];
+    __shared__ uint64_t s_pl[kFWords][6];  // bits 0-2 of the in class, then of the out class
+    __shared__ uint16_t s_vd[kFTile];      // verdicts at the tile's run starts
+    __shared__ uint16_t s_list[kFTile];    // the tile's run starts
+    __shared__ uint64_t s_seg[kFSeg + 1];
+    __shared__ uint32_t s_n;
     const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
-    const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);
-    for (uint32_t j = threadIdx.x; j < cnt; j += kFLanes) {
-        s_key[j] = G.keys[t0 + j];
-        s_lab[j] = G.lab[t0 + j];
-        s_adj[j] = G.adj[t0 + j];
+    const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);    // run starts owned
+    const uint32_t cnt_s = (uint32_t)min((uint64_t)kFStage, G.n - t0);  // tuples looked at
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const bool seg_lds = G.T <= kFSeg;
+    if (seg_lds && threadIdx.x <= (unsigned)G.T) s_seg[threadIdx.x] = G.seg[threadIdx.x];
+    if (threadIdx.x < kFWords) s_rs[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_n = 0;
+    uint64_t s0 = 1, s1 = 0;  // no segment yet
+    auto seg = [&](uint64_t i) {  // s0, s1: the segment holding tuple i
+        if (i >= s0 && i < s1) return;
+        const uint64_t *sg = seg_lds ? s_seg : G.seg;
+        int lo = 0, hi = G.T - 1;  // last t with seg[t] <= i
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sg[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        s0 = sg[lo];
+        s1 = sg[lo + 1];
+    };
+    static_assert(kFIter == 9, "adj bytes kept in two registers");
+    uint64_t ap0 = 0;  // this lane's adj bytes, for step 3 (the ninth in ap1)
+    uint32_t ap1 = 0;
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < kFIter; q++) {
+        const uint32_t j = q * kFLanes + threadIdx.x;
+        const bool valid = j < cnt_s;
+        bool st = false, nz = false;
+        uint32_t a = 0;
+        if (valid) {
+            const uint64_t i = t0 + j;
+            const uint64_t k = G.keys[i], kp = i ? G.keys[i - 1] : 0;
+            a = G.adj[i];
+            seg(i);
+            st = i == s0 || kp != k;
+            nz = k != 0;
+        }
+        if (q < 8) ap0 |= (uint64_t)a << (8 * q);
+        else ap1 = a;
+        const uint64_t mst = __ballot(st), mnz = __ballot(nz);
+        uint64_t pl[6];
+#pragma unroll
+        for (int b = 0; b < 6; b++) pl[b] = __ballot(valid && ((a >> b) & 1));
+        const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
+        const uint64_t own = jw + 64 <= cnt ? ~0ull : (jw < cnt ? (1ull << (cnt - jw)) - 1 : 0);
+        uint32_t base = 0;
+        if (lane == 0 && (mst & own)) base = atomicAdd(&s_n, (uint32_t)__popcll(mst & own));
+        base = __shfl(base, 0, kWave);
+        if (st && j < cnt) s_list[base + (uint32_t)__popcll(mst & own & lanemask_lt())] = (uint16_t)j;
+        if (lane == 0 && jw < cnt_s) {
+            const uint32_t wd = jw / kWave;
+            s_st[wd] = mst;
+            s_nz[wd] = mnz;
+#pragma unroll
+            for (int b = 0; b < 6; b++) s_pl[wd][b] = pl[b];
+            if (jw < cnt) st_bits[(t0 + jw) / 64] = mst;
+        }
     }
     __syncthreads();
-    const uint64_t prevk = t0 ? G.keys[t0 - 1] : 0;
-    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint32_t nw = (cnt_s + kWave - 1) / kWave, nruns = s_n;
     uint32_t ev = 0, z = 0;
-    uint64_t s0 = 1, s1 = 0;  // no segment yet
-    for (int q = 0; q < kFPer; q++) {
-        const uint32_t j = q * kFLanes + threadIdx.x;
-        bool st = false, rs = false;
-        if (j < cnt) {
-            const uint64_t i = t0 + j;
-            if (i < s0 || i >= s1) seg_of(G, i, s0, s1);
-            const uint64_t k = s_key[j];
-            if (i == s0 || (j ? s_key[j - 1] : prevk) != k) {
-                st = true;
-                uint32_t je = j + 1;
-                while (je < cnt && t0 + je < s1 && s_key[je] == k) je++;
-                uint64_t e = t0 + je;
-                if (je == cnt)
-                    while (e < s1 && G.keys[e] == k) e++;
-                bool zeroed = false;
-                if (e != s1) {  // the last run of a segment is never evaluated
-                    ev++;
-                    if (e <= t0 + cnt)  // inside the tile: LDS only
-                        zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t { return s_adj[t - t0]; },
-                                          [&](uint64_t t) -> uint32_t { return s_lab[t - t0]; }) == 2;
-                    else  // the run crosses the tile's end (at most one per tile): memory
-                        zeroed = eval_run(G, i, e, [&](uint64_t t) -> uint32_t { return G.adj[t]; },
-                                          [&](uint64_t t) -> uint32_t { return G.lab[t]; }) == 2;
-                    z += zeroed;
+    for (uint32_t r = threadIdx.x; r < nruns; r += kFLanes) {
+        const uint32_t j = s_list[r];
+        const uint64_t i = t0 + j;
+        seg(i);
+        // the run ends at the next start among the tuples looked at, else it continues past them
+        uint32_t w = j / kWave;
+        uint64_t b = (j % kWave) == 63 ? 0 : s_st[w] & (~0ull << (j % kWave + 1));
+        while (!b && ++w < nw) b = s_st[w];
+        const uint32_t je = b ? w * kWave + (uint32_t)__builtin_ctzll(b) : cnt_s;
+        uint64_t e = t0 + je;
+        if (je == cnt_s) {
+            const uint64_t k = G.keys[i];
+            while (e < s1 && G.keys[e] == k) e++;
+        }
+        bool zeroed = false;
+        uint32_t v = 0;
+        if (e != s1) {  // the last run of a segment is never evaluated
+            ev++;
+            if (e <= t0 + cnt_s) {
+                uint32_t in[8], out[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) in[c] = out[c] = 0;
+                for (uint32_t wd = j / kWave; wd * kWave < je; wd++) {
+                    const uint32_t lo = max(j, wd * kWave) - wd * kWave, hi = min(je, (wd + 1) * kWave) - wd * kWave;
+                    const uint64_t m = (hi == 64 ? ~0ull : ((1ull << hi) - 1)) & ~((1ull << lo) - 1);
+#pragma unroll
+                    for (int c = 0; c < 7; c++) {
+                        in[c] += class_count(s_pl[wd], m, c);
+                        out[c] += class_count(s_pl[wd] + 3, m, c);
+                    }
                 }
-                rs = k != 0 && !zeroed && e - i >= 2;
+                v = run_verdict(in, out, e - i);
+                if (v & kVZero) {
+                    zero_run(G, i, e);
+                    zeroed = true;
+                }
+            } else {  // longer than the tuples looked at: memory
+                zeroed = eval_run_mem(G, i, e) == 2;
             }
+            z += zeroed;
         }
-        const uint64_t mst = __ballot(st), mrs = __ballot(rs);
-        const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
-        if (lane == 0 && jw < cnt) {
-            const uint64_t w = (t0 + jw) / 64;
-            st_bits[w] = mst;
-            rs_bits[w] = mrs;
-        }
+        s_vd[j] = (uint16_t)((v & kVMark) ? (v & 0x1F) | (((v >> 8) & 7) << 5) | (((v >> 12) & 7) << 8) : 0);
+        if (((s_nz[j / kWave] >> (j % kWave)) & 1) && !zeroed && e - i >= 2)
+            atomicOr((unsigned long long *)&s_rs[j / kWave], 1ull << (j % kWave));
+    }
+    __syncthreads();
+    if (threadIdx.x * 64 < cnt) rs_bits[t0 / 64 + threadIdx.x] = s_rs[threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < kFIter; q++) {
+        const uint32_t j = q * kFLanes + threadIdx.x;
+        if (j >= cnt_s) continue;
+        uint32_t w = j / kWave;
+        uint64_t b = s_st[w] & ((j % kWave) == 63 ? ~0ull : ((2ull << (j % kWave)) - 1));
+        while (!b && w > 0) b = s_st[--w];
+        if (!b) continue;  // the run began in an earlier tile, which evaluated it
+        const uint32_t sj = w * kWave + 63 - (uint32_t)__builtin_clzll(b);
+        if (sj >= cnt) continue;  // a run of the next tile
+        const uint32_t c = s_vd[sj];
+        if (!(c & kVMark)) continue;
+        const uint32_t v = (c & 0x1F) | (((c >> 5) & 7) << 8) | (((c >> 8) & 7) << 12);
+        if (tuple_filtered(v, q < 8 ? (uint32_t)(ap0 >> (8 * q)) & 0xFF : ap1)) G.filtered[G.lab[t0 + j]] = 1;
     }
     block_add(ev, &G.stats[0]);
     block_add(z, &G.stats[1]);
