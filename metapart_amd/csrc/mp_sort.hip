@@ -174,9 +174,10 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
 
     uint64_t k[kSI];
     uint32_t l[kSI];
-    uint8_t a[kSI];
-    uint32_t d[kSI];
-    uint32_t rank[kSI];
+    uint32_t ap[(kSI + 3) / 4];  // adj bytes, four per register
+    uint32_t dr[kSI];            // digit << 16 (kRadix: not an item), then | rank
+#pragma unroll
+    for (int i = 0; i < (kSI + 3) / 4; i++) ap[i] = 0;
     if constexpr (GEN) {
         // reads [ra, rb] overlap the tile's windows; staged in the (idle) staging buffer
         const uint64_t ra = A.tile_r0[tile], rb = tile + 1 < A.tiles ? A.tile_r0[tile + 1] : A.G.N - 1;
@@ -203,10 +204,9 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
             const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
-            d[i] = kRadix;  // not an item
+            dr[i] = (uint32_t)kRadix << 16;  // not an item
             k[i] = 0;
             l[i] = 0;
-            a[i] = 0;
             if (idx >= A.npos) continue;
             const uint32_t q = (uint32_t)(idx - pa);
             // the read holding window idx: last j with pos[j] <= q
@@ -228,9 +228,10 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
             const uint64_t bk = canon >> A.G.bshift;
             if (bk < A.G.lo || bk >= A.G.hi) continue;
             k[i] = canon;
-            a[i] = adj_at(A.G, staged ? s_rso[lo] : A.G.seq_off[r], staged ? s_rlen[lo] : A.G.seq_len[r], p, fwd);
+            const uint8_t a = adj_at(A.G, staged ? s_rso[lo] : A.G.seq_off[r], staged ? s_rlen[lo] : A.G.seq_len[r], p, fwd);
+            ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
             l[i] = staged ? s_rlab[lo] : label_of(A.G, r);
-            d[i] = digit_of(k[i], a[i], P.shift);
+            dr[i] = digit_of(k[i], a, P.shift) << 16;
         }
     } else {
 #pragma unroll
@@ -239,41 +240,36 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
             if (idx < P.n) {
                 k[i] = P.keys_in[idx];
                 l[i] = P.lab_in[idx];
-                a[i] = P.adj_in[idx];
-                d[i] = P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a[i], P.shift);
+                const uint8_t a = P.adj_in[idx];
+                ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
+                dr[i] = (P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a, P.shift)) << 16;
             } else {
                 k[i] = 0;
                 l[i] = 0;
-                a[i] = 0;
-                d[i] = kRadix;  // invalid
+                dr[i] = (uint32_t)kRadix << 16;  // invalid
             }
         }
     }
-    uint32_t ap[(kSI + 3) / 4];  // adj bytes, four per register
-#pragma unroll
-    for (int i = 0; i < (kSI + 3) / 4; i++) ap[i] = 0;
-#pragma unroll
-    for (int i = 0; i < kSI; i++) ap[i / 4] |= (uint32_t)a[i] << (8 * (i % 4));
     // stable rank within the wave: item-major, lane-minor order
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
-        const bool valid = d[i] < (uint32_t)kRadix;
+        const uint32_t d = dr[i] >> 16;
+        const bool valid = d < (uint32_t)kRadix;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const bool bit = (d[i] >> b) & 1;
+            const bool bit = (d >> b) & 1;
             const uint64_t m = __ballot(valid && bit);
             peers &= bit ? m : ~m;
         }
-        rank[i] = 0;
         if (valid) {
             const uint32_t below = (uint32_t)__popcll(peers & lt);
             const uint32_t cnt = (uint32_t)__popcll(peers);
-            const uint32_t old = s_cnt[wave][d[i]];
-            rank[i] = old + below;
+            const uint32_t old = s_cnt[wave][d];
+            dr[i] = (d << 16) | (old + below);
             __builtin_amdgcn_wave_barrier();
-            if (below == cnt - 1) s_cnt[wave][d[i]] = (uint16_t)(old + cnt);
+            if (below == cnt - 1) s_cnt[wave][d] = (uint16_t)(old + cnt);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -315,12 +311,14 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
     // and digits are dead afterwards, which leaves registers for the look-back)
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
-        if (d[i] < (uint32_t)kRadix) {
-            rank[i] += s_cnt[wave][d[i]];
-            s_buf[rank[i]] = k[i];
-            s_dig[rank[i]] = (uint8_t)d[i];
+        const uint32_t d = dr[i] >> 16;
+        if (d < (uint32_t)kRadix) {
+            const uint32_t rk = (dr[i] & 0xFFFF) + s_cnt[wave][d];
+            s_buf[rk] = k[i];
+            s_dig[rk] = (uint8_t)d;
+            dr[i] = rk;  // the rank from here on
         } else {
-            rank[i] = ~0u;
+            dr[i] = ~0u;
         }
     }
     if (tid < kRadix) {
@@ -361,9 +359,9 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
     uint8_t *s_adj = (uint8_t *)(s_lab + kTile);
 #pragma unroll
     for (int i = 0; i < kSI; i++) {
-        if (rank[i] != ~0u) {
-            s_lab[rank[i]] = l[i];
-            s_adj[rank[i]] = (uint8_t)(ap[i / 4] >> (8 * (i % 4)));
+        if (dr[i] != ~0u) {
+            s_lab[dr[i]] = l[i];
+            s_adj[dr[i]] = (uint8_t)(ap[i / 4] >> (8 * (i % 4)));
         }
     }
     __syncthreads();
