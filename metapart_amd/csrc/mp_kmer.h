@@ -48,22 +48,24 @@ __device__ __forceinline__ uint8_t adj_of(uint8_t prev, uint8_t next)
 }
 __device__ __forceinline__ uint32_t code_of(uint8_t c) { return (0x2310u >> (((c & 6) >> 1) * 4)) & 3; }
 
-// Canonical k-mer of the window starting at base b (position p of a read with
-// sequence at fq[so], length L).  Returns false if the window holds an 'N'.
-__device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
+// Canonical k-mer of the window starting at base b (position p of its read),
+// the two streams read through word accessors (memory or a staged copy).
+// Returns false if the window holds an 'N'.
+template <typename NWord, typename PWord>
+__device__ __forceinline__ bool kmer_from(int K, uint64_t b, uint32_t p, NWord nword, PWord pword, uint64_t &canon,
+                                          bool &fwd)
 {
-    const int K = P.K;
     {
         const uint64_t w = b >> 6;
         const int o = (int)(b & 63);
-        const uint64_t hi = P.nmask[w], lo = P.nmask[w + 1];
+        const uint64_t hi = nword(w), lo = nword(w + 1);
         const int s = o + K;
         const uint64_t v = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
         if (v & ((1ull << K) - 1)) return false;
     }
     const uint64_t w = b >> 5;
     const int o = (int)(b & 31);
-    const uint64_t hi = P.packed[w], lo = P.packed[w + 1];
+    const uint64_t hi = pword(w), lo = pword(w + 1);
     const int s = 2 * (o + K);
     uint64_t F = s <= 64 ? (hi >> (64 - s)) : ((hi << (s - 64)) | (lo >> (128 - s)));
     if (K < 32) F &= (1ull << (2 * K)) - 1;
@@ -72,6 +74,18 @@ __device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p
     fwd = F < R;
     canon = fwd ? F : R;
     return true;
+}
+
+__device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
+{
+    return kmer_from(P.K, b, p, [&](uint64_t w) { return P.nmask[w]; }, [&](uint64_t w) { return P.packed[w]; },
+                     canon, fwd);
+}
+
+// adj byte from the bytes before and after the window (enumerateKmers.c:215, 239)
+__device__ __forceinline__ uint8_t adj_from(uint8_t prev, uint8_t next, bool fwd)
+{
+    return fwd ? adj_of(prev, next) : adj_of(rc_char(next), rc_char(prev));
 }
 
 // adj byte of the window at p of a read (sequence at fq[so], length L)

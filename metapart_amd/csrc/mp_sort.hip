@@ -129,7 +129,7 @@ struct GenPass {
     uint64_t tiles;
     uint64_t npos;            // windows of all reads
 };
-constexpr int kGenReads = 1024;  // reads of a tile staged in LDS (more: read from memory)
+constexpr int kGenReads = 256;  // reads of a tile staged in LDS (more: everything read from memory)
 
 // tile_r0[t] = the read holding window t * tile: last r with pos_off[r] <= t * tile
 __global__ void tile_read_kernel(const uint64_t *__restrict__ pos_off, uint64_t N, uint64_t tile, uint64_t tiles,
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
 {
     constexpr int kSW = kSB / kWave;
     constexpr int kTile = kSB * kSI;
-    static_assert(!GEN || 2 * kGenReads * 8 + (3 * kGenReads + 1) * 4 <= kTile * 8, "read staging exceeds the tile");
+    static_assert(!GEN || 2 * kGenReads * 8 + (3 * kGenReads + 2) * 4 <= kTile * 8, "read staging exceeds the tile");
     // one staging buffer (keys, then labels + adj) keeps LDS under half a CU,
     // so two tiles run per CU and one tile's memory phases overlap the other's
     // ranking and look-back
@@ -179,14 +179,24 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
 #pragma unroll
     for (int i = 0; i < (kSI + 3) / 4; i++) ap[i] = 0;
     if constexpr (GEN) {
-        // reads [ra, rb] overlap the tile's windows; staged in the (idle) staging buffer
+        // reads [ra, rb] overlap the tile's windows.  When they fit, their table,
+        // their 2-bit and 'N' words and their FASTQ sequence bytes are staged in
+        // the (idle) staging buffer, so the items read LDS only; else memory.
         const uint64_t ra = A.tile_r0[tile], rb = tile + 1 < A.tiles ? A.tile_r0[tile + 1] : A.G.N - 1;
         const uint32_t nr = (uint32_t)(rb - ra + 1);
-        const bool staged = nr <= (uint32_t)kGenReads;
+        const uint64_t pa = A.G.pos_off[ra];
+        const uint64_t b0 = A.G.base_off[ra], b1 = A.G.base_off[rb + 1];   // bases of the reads
+        const uint64_t f0 = A.G.seq_off[ra], f1 = A.G.seq_off[rb] + A.G.seq_len[rb];  // their FASTQ span
+        const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
+        const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
+        const uint64_t fw0 = f0 >> 2, nf = (f1 + 3) / 4 - fw0;
+        constexpr uint32_t kTab = 2 * 8 * kGenReads + 4 * (3 * kGenReads + 2);  // bytes of the read table
+        const bool staged = nr <= (uint32_t)kGenReads && kTab + 8 * (np + nn) + 4 * nf <= 8u * kTile;
         uint64_t *s_rbase = s_buf, *s_rso = s_buf + kGenReads;
         uint32_t *s_rpos = (uint32_t *)(s_buf + 2 * kGenReads), *s_rlen = s_rpos + kGenReads + 1,
                  *s_rlab = s_rlen + kGenReads;
-        const uint64_t pa = A.G.pos_off[ra];
+        uint64_t *s_pw = s_buf + kTab / 8, *s_nw = s_pw + np;
+        uint32_t *s_fw = (uint32_t *)(s_nw + nn);
         if (staged) {
             for (uint32_t j = tid; j <= nr; j += kSB) {
                 s_rpos[j] = (uint32_t)(A.G.pos_off[ra + j] - pa);
@@ -197,9 +207,18 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
                     s_rlab[j] = label_of(A.G, ra + j);
                 }
             }
+            for (uint32_t j = tid; j < np; j += kSB) s_pw[j] = A.G.packed[wp0 + j];
+            for (uint32_t j = tid; j < nn; j += kSB) s_nw[j] = A.G.nmask[wn0 + j];
+            const uint32_t *fq32 = (const uint32_t *)A.G.fq;  // 4-aligned words inside the records
+            for (uint32_t j = tid; j < nf; j += kSB) s_fw[j] = fq32[fw0 + j];
         }
         __syncthreads();
         auto rpos = [&](uint32_t j) -> uint32_t { return staged ? s_rpos[j] : (uint32_t)(A.G.pos_off[ra + j] - pa); };
+        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_nw[w - wn0] : A.G.nmask[w]; };
+        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_pw[w - wp0] : A.G.packed[w]; };
+        auto fqb = [&](uint64_t x) -> uint8_t {
+            return staged ? (uint8_t)(s_fw[(x >> 2) - fw0] >> (8 * (x & 3))) : A.G.fq[x];
+        };
         uint32_t lo = 0;  // read of the lane's last item: its items are 64 windows apart, so a few steps on
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
@@ -220,15 +239,19 @@ __global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P
                 }
             }
             const uint64_t r = ra + lo;
-            const uint32_t p = q - (staged ? s_rpos[lo] : (uint32_t)(A.G.pos_off[r] - pa));
+            const uint32_t p = q - rpos(lo);
             const uint64_t rbase = staged ? s_rbase[lo] : A.G.base_off[r];
             uint64_t canon;
             bool fwd;
-            if (!kmer_at(A.G, rbase + p, p, canon, fwd)) continue;
+            if (!kmer_from(A.G.K, rbase + p, p, nword, pword, canon, fwd)) continue;
             const uint64_t bk = canon >> A.G.bshift;
             if (bk < A.G.lo || bk >= A.G.hi) continue;
             k[i] = canon;
-            const uint8_t a = adj_at(A.G, staged ? s_rso[lo] : A.G.seq_off[r], staged ? s_rlen[lo] : A.G.seq_len[r], p, fwd);
+            const uint64_t so = staged ? s_rso[lo] : A.G.seq_off[r];
+            const uint32_t L = staged ? s_rlen[lo] : A.G.seq_len[r];
+            const uint8_t prev = p == 0 ? (uint8_t)'N' : fqb(so + p - 1);
+            const uint8_t next = p == L - (uint32_t)A.G.K ? (uint8_t)'N' : fqb(so + p + A.G.K);
+            const uint8_t a = adj_from(prev, next, fwd);
             ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
             l[i] = staged ? s_rlab[lo] : label_of(A.G, r);
             dr[i] = digit_of(k[i], a, P.shift) << 16;
