@@ -53,6 +53,7 @@ constexpr int kHLowBits = 20 - kHDigitBits;
 constexpr int kHLow = 1 << kHLowBits;
 constexpr uint32_t kHReads = 1024;       // reads per tile (steps 1, 2)
 constexpr uint64_t kHSlab = 4u << 20;    // items per workgroup (step 3)
+constexpr int kHWords = 1024;            // staged stream words of a group of kRG reads (150 bp: ~450)
 
 constexpr int kBhRep = 32;  // copies of the byte histograms (tile t adds to copy t % kBhRep)
 
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint32_t s_dig[kHDigits];
     __shared__ uint64_t s_out[kHDigits];
     __shared__ uint32_t s_bh[PHASE == 0 ? 8 : 1][kSortRadix];
+    __shared__ uint64_t s_words[kHWords];  // the group's 2-bit words, then its 'N' words
     const int tid = threadIdx.x;
     if (PHASE == 0)
         for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&s_bh[0][0])[j] = 0;
@@ -104,7 +106,18 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         const uint64_t p0 = P.pos_off[r0];
         if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
         if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        // stage the group's stream words (a group of very long reads reads memory)
+        const uint64_t b0 = P.base_off[r0], b1 = P.base_off[r0 + nr];
+        const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
+        const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
+        const bool staged = np + nn <= (uint64_t)kHWords;
+        if (staged) {
+            for (uint32_t j = tid; j < np; j += kKBlock) s_words[j] = P.packed[wp0 + j];
+            for (uint32_t j = tid; j < nn; j += kKBlock) s_words[np + j] = P.nmask[wn0 + j];
+        }
         __syncthreads();
+        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
+        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
         const uint32_t total = s_pref[nr];
         for (uint32_t q = tid; q < total; q += kKBlock) {
             int lo = 0, h = nr - 1;
@@ -115,7 +128,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             const uint32_t p = q - s_pref[lo];
             uint64_t canon;
             bool fwd;
-            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            if (!kmer_from(P.K, s_base[lo] + p, p, nword, pword, canon, fwd)) continue;
             const uint32_t b = (uint32_t)(canon >> P.bshift);
             const uint32_t d = b >> kHLowBits;
             const uint32_t slot = atomicAdd(&s_dig[d], 1u);
@@ -470,6 +483,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
     __shared__ uint32_t h[8][kSortRadix];
     __shared__ uint32_t s_pref[kRG + 1];
     __shared__ uint64_t s_base[kRG];
+    __shared__ uint64_t s_words[kHWords];  // the group's 2-bit words, then its 'N' words
     const int tid = threadIdx.x;
     for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&h[0][0])[j] = 0;
     const uint64_t ngrp = (P.N + kRG - 1) / kRG;
@@ -480,7 +494,17 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
         const uint64_t p0 = P.pos_off[r0];
         if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
         if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        const uint64_t b0 = P.base_off[r0], b1 = P.base_off[r0 + nr];
+        const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
+        const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
+        const bool staged = np + nn <= (uint64_t)kHWords;
+        if (staged) {
+            for (uint32_t j = tid; j < np; j += kKBlock) s_words[j] = P.packed[wp0 + j];
+            for (uint32_t j = tid; j < nn; j += kKBlock) s_words[np + j] = P.nmask[wn0 + j];
+        }
         __syncthreads();
+        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
+        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
         const uint32_t total = s_pref[nr];
         for (uint32_t q = tid; q < total; q += kKBlock) {
             int lo = 0, hi = nr - 1;
@@ -491,7 +515,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
             const uint32_t p = q - s_pref[lo];
             uint64_t canon;
             bool fwd;
-            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            if (!kmer_from(P.K, s_base[lo] + p, p, nword, pword, canon, fwd)) continue;
             const uint64_t bk = canon >> P.bshift;
             if (bk < P.lo || bk >= P.hi) continue;
 #pragma unroll
