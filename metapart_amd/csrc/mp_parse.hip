@@ -144,7 +144,7 @@ __global__ void rec_finalize_kernel(const uint8_t *__restrict__ fq, uint64_t n_r
 // dense base array (one wave per read, consecutive lanes on consecutive
 // bytes), then each lane turns 64 dense bases (four aligned 16-byte loads)
 // into two words of the 2-bit stream and one word of the N stream.
-__global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restrict__ fq,
+__global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restrict__ fq, uint64_t nbytes,
                                                           const uint64_t *__restrict__ seq_off,
                                                           const uint64_t *__restrict__ base_off, uint64_t n_rec,
                                                           uint8_t *__restrict__ dense)
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restr
     const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / kWave;
     for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n_rec; r += waves) {
         const uint64_t src = seq_off[r], dst = base_off[r], len = base_off[r + 1] - dst;
-        for (uint64_t b = __lane_id(); b < len; b += kWave) dense[dst + b] = fq[src + b];
+        wave_copy(dense + dst, fq + src, len, fq + nbytes, __lane_id());
     }
 }
 
@@ -311,7 +311,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     MP_TRY(ensure(ctx, ctx->keys[1], nthreads * 64 + 64));
     uint8_t *dense = (uint8_t *)ctx->keys[1].p;
     if (N) {
-        hipLaunchKernelGGL(dense_bases_kernel, dim3(wave_grid(N)), dim3(256), 0, ctx->stream, fq,
+        hipLaunchKernelGGL(dense_bases_kernel, dim3(wave_grid(N)), dim3(256), 0, ctx->stream, fq, ctx->fq_bytes,
                            (const uint64_t *)ctx->seq_off.p, (const uint64_t *)ctx->base_off.p, N, dense);
         MP_HIP(hipGetLastError());
     }

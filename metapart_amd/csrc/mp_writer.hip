@@ -114,7 +114,7 @@ __global__ void gather_len_kernel(const uint32_t *__restrict__ slot, const uint3
 }
 
 // one wave per item (grid-stride over items): lanes copy consecutive bytes
-__global__ void copy_items_kernel(const uint8_t *__restrict__ fq, const uint64_t *__restrict__ rec_off,
+__global__ void copy_items_kernel(const uint8_t *__restrict__ fq, uint64_t nbytes, const uint64_t *__restrict__ rec_off,
                                   const uint32_t *__restrict__ slot, const uint64_t *__restrict__ off, uint64_t n_items,
                                   uint8_t *__restrict__ out)
 {
@@ -125,7 +125,7 @@ __global__ void copy_items_kernel(const uint8_t *__restrict__ fq, const uint64_t
         const uint64_t src = rec_off[r];
         const uint64_t dst = off[item];
         const uint64_t len = off[item + 1] - dst;
-        for (uint64_t b = lane; b < len; b += kWave) out[dst + b] = fq[src + b];
+        wave_copy(out + dst, fq + src, len, fq + nbytes, lane);
     }
 }
 
@@ -248,7 +248,7 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     const uint64_t real_items = fpos[F];
     if (real_items) {
         hipLaunchKernelGGL(copy_items_kernel, dim3(wave_grid(real_items)), dim3(256), 0, ctx->stream,
-                           ctx->d_fq, (const uint64_t *)ctx->rec_off.p, slot, (const uint64_t *)off, real_items,
+                           ctx->d_fq, ctx->fq_bytes, (const uint64_t *)ctx->rec_off.p, slot, (const uint64_t *)off, real_items,
                            (uint8_t *)ctx->out.p);
         MP_HIP(hipGetLastError());
     }
