@@ -147,7 +147,9 @@ __global__ void tile_read_kernel(const uint64_t *__restrict__ pos_off, uint64_t 
 }
 
 template <int kSB, int kSI, bool GEN>
-__global__ __launch_bounds__(kSB, 2 * kSB / 256) void onesweep_kernel(SortPass P, GenPass A)  // two tiles per CU
+// waves per SIMD the tile shape's LDS allows: two 1024-lane tiles, or four 512 x 6 tiles, per CU
+__global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) void onesweep_kernel(SortPass P,
+                                                                                                      GenPass A)
 {
     constexpr int kSW = kSB / kWave;
     constexpr int kTile = kSB * kSI;
@@ -401,7 +403,9 @@ static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, const So
     if (!grid_fits(tiles, SB))
         return ctx->fail(MP_EINVAL, "radix pass of %llu tiles exceeds one dispatch: use more scans",
                          (unsigned long long)tiles);
-    if (SB == 512 && SI == 8)
+    if (SB == 512 && SI == 6)
+        hipLaunchKernelGGL((onesweep_kernel<512, 6, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
+    else if (SB == 512 && SI == 8)
         hipLaunchKernelGGL((onesweep_kernel<512, 8, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
     else if (SB == 512 && SI == 12)
         hipLaunchKernelGGL((onesweep_kernel<512, 12, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
@@ -477,6 +481,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     if (const char *e = getenv("MP_SORT_TILE")) shape = atoi(e);
     int SB = 1024, SI = 6;
     switch (shape) {
+    case 3072: SB = 512; SI = 6; break;
     case 4096: SB = 512; SI = 8; break;
     case 6144: SB = 512; SI = 12; break;
     case 8192: SB = 1024; SI = 8; break;

@@ -20,7 +20,7 @@ TUPLE_BYTES = 13           # k-mer u64 + label u32 + adj u8
 PER_TUPLE_DEVICE = 2 * TUPLE_BYTES + 1   # ping-pong buffers, sort status, filter/union bitmaps
 PER_READ_DEVICE = 70       # record table, labels, flags, sizes, pair-step arrays
 PER_READ_WRITER = 68       # two output items per record (sorted ping-pong) and their offsets
-TILE_SHAPES = (0, 8192, 4096, 6144)  # 0: 1024 lanes x 6 tuples
+TILE_SHAPES = (0, 8192, 4096, 6144, 3072)  # 0: 1024 lanes x 6 tuples
 
 
 def resident_bytes(reads: int, fq_bytes: int, bases: int, kmers: int) -> int:
