@@ -30,6 +30,10 @@ __device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ uint64_t lanemask_lt()
 {
     const unsigned lane = __lane_id();
@@ -131,6 +135,7 @@ struct mp_ctx {
     std::string err;
     uint64_t budget = 0, held = 0;
     uint16_t epoch = 0;  // look-back status epoch of the radix sort
+    int status_fmt = 0;  // 1: 32-bit status words, 2: 64-bit (the array is cleared on a change)
 
     // reads
     const uint8_t *d_fq = nullptr;  // FASTQ bytes (owned if fq_owned)

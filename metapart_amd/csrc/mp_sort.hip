@@ -55,12 +55,28 @@ constexpr uint64_t kStAgg = 1, kStInc = 2;
 #define MP_LOOKBACK 4
 #endif
 constexpr int kLookBack = MP_LOOKBACK;  // predecessor tiles read per look-back step
-constexpr uint64_t kStVal = (1ull << 46) - 1;
-
-__device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint64_t v)
-{
-    return ((uint64_t)epoch << 48) | (flag << 46) | v;
-}
+// Look-back status words: epoch | flag (none, aggregate, inclusive) | value.
+// 64-bit: 16-bit epoch, 46-bit value.  32-bit: 4-bit epoch, 26-bit value,
+// half the look-back traffic; used when no digit of the pass has 2^26 tuples.
+template <bool S32> struct Status;
+template <> struct Status<false> {
+    using T = uint64_t;
+    __device__ static T pack(uint32_t epoch, uint64_t flag, uint64_t v) { return ((uint64_t)epoch << 48) | (flag << 46) | v; }
+    __device__ static uint32_t flag(T s, uint32_t epoch)
+    {
+        return (uint32_t)(s >> 48) == epoch ? (uint32_t)((s >> 46) & 3) : 0u;
+    }
+    __device__ static uint64_t val(T s) { return s & ((1ull << 46) - 1); }
+};
+template <> struct Status<true> {
+    using T = uint32_t;
+    __device__ static T pack(uint32_t epoch, uint64_t flag, uint64_t v)
+    {
+        return ((epoch & 15u) << 28) | ((uint32_t)flag << 26) | (uint32_t)v;
+    }
+    __device__ static uint32_t flag(T s, uint32_t epoch) { return (s >> 28) == (epoch & 15u) ? (s >> 26) & 3u : 0u; }
+    __device__ static uint64_t val(T s) { return s & ((1u << 26) - 1); }
+};
 
 struct SortPass {
     const uint64_t *keys_in;
@@ -74,7 +90,7 @@ struct SortPass {
     const uint64_t *grp_first;  // n_grp + 1 first read ids of the samples
     int n_grp;
     const uint64_t *digit_base;
-    uint64_t *status;
+    void *status;
     uint32_t *tile_ctr;
     uint32_t epoch;
 };
@@ -146,8 +162,8 @@ __global__ void tile_read_kernel(const uint64_t *__restrict__ pos_off, uint64_t 
     tile_r0[t] = lo;
 }
 
-template <int kSB, int kSI, bool GEN>
 // waves per SIMD the tile shape's LDS allows: two 1024-lane tiles, or four 512 x 6 tiles, per CU
+template <int kSB, int kSI, bool GEN, bool S32>
 __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) void onesweep_kernel(SortPass P,
                                                                                                       GenPass A)
 {
@@ -165,6 +181,8 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
     __shared__ uint32_t s_wsum[kRadix / kWave];
     __shared__ uint32_t s_tile, s_nvalid;
 
+    using S = Status<S32>;
+    typename S::T *const status = (typename S::T *)P.status;
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     const unsigned lane = __lane_id();
@@ -311,7 +329,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             run += c;
         }
         total = run;
-        st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, tile == 0 ? kStInc : kStAgg, total));
+        st_agent(status + tile * kRadix + tid, S::pack(P.epoch, tile == 0 ? kStInc : kStAgg, total));
         uint32_t inc = total;
         for (int o = 1; o < kWave; o <<= 1) {
             const uint32_t v = __shfl_up(inc, o, kWave);
@@ -355,22 +373,25 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             int64_t j = (int64_t)tile - 1;
             bool done = false;
             while (!done) {
-                uint64_t w[kLookBack];
+                typename S::T w[kLookBack];
 #pragma unroll
                 for (int q = 0; q < kLookBack; q++)
-                    w[q] = j - q >= 0 ? ld_agent(P.status + (uint64_t)(j - q) * kRadix + tid) : 0;
+                    w[q] = j - q >= 0 ? ld_agent(status + (uint64_t)(j - q) * kRadix + tid) : 0;
 #pragma unroll
                 for (int q = 0; q < kLookBack; q++) {
                     if (done || j - q < 0) break;
-                    uint64_t s = w[q];
-                    while ((uint32_t)(s >> 48) != P.epoch || ((s >> 46) & 3) == 0)  // not published yet
-                        s = ld_agent(P.status + (uint64_t)(j - q) * kRadix + tid);
-                    prefix += s & kStVal;
-                    done = ((s >> 46) & 3) == kStInc;
+                    typename S::T s = w[q];
+                    uint32_t f = S::flag(s, P.epoch);
+                    while (!f) {  // not published yet
+                        s = ld_agent(status + (uint64_t)(j - q) * kRadix + tid);
+                        f = S::flag(s, P.epoch);
+                    }
+                    prefix += S::val(s);
+                    done = f == kStInc;
                 }
                 j -= kLookBack;
             }
-            st_agent(P.status + tile * kRadix + tid, st_pack(P.epoch, kStInc, prefix + total));
+            st_agent(status + tile * kRadix + tid, S::pack(P.epoch, kStInc, prefix + total));
         }
         s_gbase[tid] = P.digit_base[tid] + prefix - ds;  // output index of tile position 0 of this digit
     }
@@ -397,24 +418,33 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
     }
 }
 
+template <bool GEN, bool S32>
+static int launch_shape(mp_ctx *ctx, int SB, int SI, uint64_t tiles, const SortPass &P, const GenPass &A)
+{
+    if (SB == 512 && SI == 6)
+        hipLaunchKernelGGL((onesweep_kernel<512, 6, GEN, S32>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
+    else if (SB == 512 && SI == 8)
+        hipLaunchKernelGGL((onesweep_kernel<512, 8, GEN, S32>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
+    else if (SB == 512 && SI == 12)
+        hipLaunchKernelGGL((onesweep_kernel<512, 12, GEN, S32>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P,
+                           A);
+    else if (SB == 1024 && SI == 8)
+        hipLaunchKernelGGL((onesweep_kernel<1024, 8, GEN, S32>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P,
+                           A);
+    else
+        hipLaunchKernelGGL((onesweep_kernel<1024, 6, GEN, S32>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P,
+                           A);
+    MP_HIP(hipGetLastError());
+    return 0;
+}
+
 template <bool GEN>
-static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, const SortPass &P, const GenPass &A)
+static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, bool s32, const SortPass &P, const GenPass &A)
 {
     if (!grid_fits(tiles, SB))
         return ctx->fail(MP_EINVAL, "radix pass of %llu tiles exceeds one dispatch: use more scans",
                          (unsigned long long)tiles);
-    if (SB == 512 && SI == 6)
-        hipLaunchKernelGGL((onesweep_kernel<512, 6, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
-    else if (SB == 512 && SI == 8)
-        hipLaunchKernelGGL((onesweep_kernel<512, 8, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
-    else if (SB == 512 && SI == 12)
-        hipLaunchKernelGGL((onesweep_kernel<512, 12, GEN>), dim3((unsigned)tiles), dim3(512), 0, ctx->stream, P, A);
-    else if (SB == 1024 && SI == 8)
-        hipLaunchKernelGGL((onesweep_kernel<1024, 8, GEN>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P, A);
-    else
-        hipLaunchKernelGGL((onesweep_kernel<1024, 6, GEN>), dim3((unsigned)tiles), dim3(1024), 0, ctx->stream, P, A);
-    MP_HIP(hipGetLastError());
-    return 0;
+    return s32 ? launch_shape<GEN, true>(ctx, SB, SI, tiles, P, A) : launch_shape<GEN, false>(ctx, SB, SI, tiles, P, A);
 }
 
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
@@ -447,7 +477,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     }
     if (!d_grp_first) h[9 * kRadix] = n;  // no samples: one constant digit
     // digit base offsets of the passes that do work
-    uint64_t base[kPasses][kRadix];
+    uint64_t base[kPasses][kRadix], dmax[kPasses];
     bool run[kPasses];
     for (int p = 0; p < kPasses; p++) {
         uint64_t acc = 0, mx = 0;
@@ -460,6 +490,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
             return ctx->fail(MP_EDEVICE, "sort histogram total %llu != %llu", (unsigned long long)acc,
                              (unsigned long long)n);
         run[p] = ((pass_mask >> p) & 1) && mx != n;
+        dmax[p] = mx;
     }
     // the generating pass: the first pass that does work, else the lowest of the mask
     int gen_p = -1;
@@ -513,9 +544,14 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     int executed = 0;
     for (int p = 0; p < kPasses; p++) {
         if (!run[p]) continue;
-        if (++ctx->epoch == 0) {  // wrapped: old status words could alias
+        // status words: 32-bit when every digit's count fits 26 bits; the array is
+        // cleared when the format changes or the epoch wraps (4 bits for 32-bit words)
+        const bool s32 = dmax[p] < (1ull << 26);
+        ++ctx->epoch;
+        if (ctx->epoch == 0 || ctx->status_fmt != (s32 ? 1 : 2) || (s32 && (ctx->epoch & 15) == 0)) {
             MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
-            ctx->epoch = 1;
+            if (ctx->epoch == 0) ctx->epoch = 1;
+            ctx->status_fmt = s32 ? 1 : 2;
         }
         const int src = *cur, dst = 1 - *cur;
         SortPass P;
@@ -530,14 +566,14 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.grp_first = d_grp_first;
         P.n_grp = n_grp;
         P.digit_base = d_base + p * kRadix;
-        P.status = (uint64_t *)ctx->status.p;
+        P.status = ctx->status.p;
         P.tile_ctr = tile_ctr + executed;
         P.epoch = ctx->epoch;
         hipEventRecord(ctx->pev[2 + 2 * executed], ctx->stream);
         if (p == gen_p)
-            MP_TRY(launch_onesweep<true>(ctx, SB, SI, gtiles, P, A));
+            MP_TRY(launch_onesweep<true>(ctx, SB, SI, gtiles, s32, P, A));
         else
-            MP_TRY(launch_onesweep<false>(ctx, SB, SI, tiles, P, A));
+            MP_TRY(launch_onesweep<false>(ctx, SB, SI, tiles, s32, P, A));
         hipEventRecord(ctx->pev[3 + 2 * executed], ctx->stream);
         *cur = dst;
         executed++;
