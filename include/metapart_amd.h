@@ -70,8 +70,8 @@ typedef struct {
     int k;                /* k-mer length, 11..32 */
     int min3;             /* 0: every k-mer; 1: three window minimizers per read */
     uint64_t hbm_budget;  /* device bytes this ctx may hold; 0 = 90 % of free memory */
-    int sort_digit_bits;  /* radix digit width 8 or 11; 0 = autotuned default */
-    int sort_tile;        /* keys per sort tile (multiple of 1024); 0 = autotuned */
+    int sort_digit_bits;  /* radix digit width: 0 or 8 (the only width; anything else is MP_EINVAL) */
+    int sort_tile;        /* tuples per radix tile: 0 (1024 lanes x 6), 3072, 4096, 6144 (512 lanes) or 8192 */
 } mp_cfg;
 
 typedef struct {

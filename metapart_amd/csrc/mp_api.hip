@@ -17,6 +17,10 @@ int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
     if (!cfg || !out) return MP_EINVAL;
     *out = nullptr;
     if (cfg->k <= 10 || cfg->k > 32) return MP_EINVAL;
+    if (cfg->sort_digit_bits != 0 && cfg->sort_digit_bits != 8) return MP_EINVAL;  // 8-bit digits only
+    if (cfg->sort_tile != 0 && cfg->sort_tile != 3072 && cfg->sort_tile != 4096 && cfg->sort_tile != 6144 &&
+        cfg->sort_tile != 8192)
+        return MP_EINVAL;
     mp_ctx *ctx = new (std::nothrow) mp_ctx();
     if (!ctx) return MP_ENOMEM;
     ctx->cfg = *cfg;

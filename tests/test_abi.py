@@ -41,6 +41,18 @@ def test_ctx_create_without_gpu_fails_cleanly():
         mp.Context(k=31)
 
 
+def test_ctx_create_rejects_unsupported_sort_shapes():
+    # checked before any device call: an unsupported digit width or tile shape is MP_EINVAL on any machine
+    import ctypes as ct
+    for kw in (dict(sort_digit_bits=11), dict(sort_digit_bits=4), dict(sort_tile=5000)):
+        cfg = mp.Cfg(device=0, k=31, min3=0, hbm_budget=0, sort_digit_bits=0, sort_tile=0)
+        for f, v in kw.items():
+            setattr(cfg, f, v)
+        h = ct.c_void_p()
+        assert mp.lib().mp_ctx_create(ct.byref(cfg), ct.byref(h)) == -1
+        assert not h.value
+
+
 @pytest.mark.parametrize("case", [c for c in golden()["cases"]], ids=[c["name"] for c in golden()["cases"]])
 def test_index_tool_matches_reference(case, tmp_path):
     d = str(tmp_path)
