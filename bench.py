@@ -255,7 +255,8 @@ def c3_section(args, mp, torch):
     out = {"workload": "%d SE reads x 150 bp, 1000 genomes 0.5-8 Mb, log-normal(1.5), 1%% error, k=%d, 64 chunks, "
                        "T=1, scans from the HBM budget" % (ctx.N, args.k),
            "reads": ctx.N, "fastq_bytes": buf.nbytes, "kmers": kmers, "scans": S,
-           "scan_tuples": [int(x["tuples"]) for x in r.scans], "budget_bytes": budget, "planned_bytes": need,
+           "scan_tuples": [int(x["tuples"]) for x in r.scans], "scan_parts": [int(x["parts"]) for x in r.scans],
+           "budget_bytes": budget, "planned_bytes": need,
            "device_bytes": ctx.device_bytes, "ms_load_from_host": 1e3 * t_load, "ms_partition": 1e3 * t_part,
            "reads_per_s": ctx.N / t_part, "reads_per_s_with_upload": ctx.N / (t_part + t_load),
            "stages_ms": [{k: x[k] for k in ("ms_enumerate", "ms_sort", "ms_filter", "ms_union")} for x in r.scans],

@@ -87,6 +87,7 @@ typedef struct {
     float ms_sort_local;      /* the local pass ranking tuples inside high-bit groups (0: all passes global) */
     uint64_t sort_long_groups;/* groups too long for the local pass, sorted by their own global passes */
     float ms_sort_gen;        /* the first radix pass, generating the tuples from the reads (0: enumerated) */
+    int parts;                /* 0: one pass; else the scan did not fit the budget and went twice over this many bucket parts */
 } mp_scan_stats;
 
 typedef struct {

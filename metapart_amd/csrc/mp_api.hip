@@ -1,5 +1,7 @@
 // C-ABI entry points of libmetapart_amd (declared in include/metapart_amd.h).
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <thread>
@@ -59,7 +61,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
                         &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->khist, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->khist, &ctx->labsnap, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
                         &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
                         &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
                         &ctx->iaux[0], &ctx->iaux[1]};
@@ -230,6 +232,106 @@ int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const u
     return load_common(ctx, chunk_ofs, chunk_first_id, C);
 }
 
+// Device bytes a scan holds per tuple: two tuple buffers (13 B each), the
+// zeroed-run and run bitmaps, the look-back status, and the growth slack of
+// the buffers.
+constexpr uint64_t kScanBytesPerTuple = 29;
+
+// Tuples a scan can hold within the budget: the bytes not held by anything
+// else, less a label snapshot and the room the pair step and the writer will
+// need, over kScanBytesPerTuple (MP_SCAN_PART_TUPLES lowers it, for tests).
+static uint64_t scan_tuple_room(mp_ctx *ctx)
+{
+    const uint64_t N = ctx->N;
+    const uint64_t tuple_held = ctx->keys[0].bytes + ctx->keys[1].bytes + ctx->lab[0].bytes + ctx->lab[1].bytes +
+                                ctx->adj[0].bytes + ctx->adj[1].bytes + ctx->zbits.bytes + ctx->runbits.bytes +
+                                ctx->status.bytes;
+    const uint64_t other = ctx->held - tuple_held;
+    const uint64_t label_bytes = ctx->labsnap.bytes ? 0 : N * sizeof(uint32_t) + 64;
+    const uint64_t later_held = ctx->gparent.bytes + ctx->gsize.bytes + ctx->pairc.bytes + ctx->items[0].bytes +
+                                ctx->items[1].bytes + ctx->ilens[0].bytes + ctx->ilens[1].bytes + ctx->iaux[0].bytes +
+                                ctx->iaux[1].bytes + ctx->out.bytes;
+    // (measured: 48 B per read for the pair step, 66 B per read of items plus
+    // the output for the writer; with the buffers' growth slack of 1/16)
+    const uint64_t later_need = (128 * N + ctx->fq_bytes) / 16 * 17 + (1u << 20);
+    const uint64_t reserve = later_need > later_held ? later_need - later_held : 0;
+    // the generating pass's look-back status covers every k-mer window, not only the scan's
+    const uint64_t npos_status = ctx->total_bases / 16;
+    const uint64_t fixed = other + label_bytes + reserve + npos_status;
+    const uint64_t avail = ctx->budget > fixed ? ctx->budget - fixed : 0;
+    uint64_t cap = avail / kScanBytesPerTuple;
+    if (const char *e = getenv("MP_SCAN_PART_TUPLES")) cap = std::min<uint64_t>(cap, strtoull(e, nullptr, 10));
+    return cap;
+}
+
+// The two-sweep scan: bucket parts of at most cap tuples, cut where the
+// per-bucket counts allow; the labels are fixed at the scan's start.
+static int scan_two_sweeps(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bounds, int T,
+                           uint64_t expected, uint64_t cap, mp_scan_stats *st)
+{
+    if (cap == 0) return ctx->fail(MP_ENOMEM, "the device budget leaves no room for a scan's tuples");
+    if (ctx->cfg.min3)  // the window state machine emits a different set over a bucket sub-range (C.9)
+        return ctx->fail(MP_ENOMEM, "scan %d needs %llu tuples, more than the budget's %llu, and 3-minimizer scans "
+                                    "cannot be split into bucket parts", s, (unsigned long long)expected,
+                         (unsigned long long)cap);
+    std::vector<uint64_t> pre;
+    MP_TRY(bucket_prefix_range(ctx, lo, hi, pre));
+    if (pre.back() != expected)
+        return ctx->fail(MP_ECOUNT, "scan %d holds %llu tuples, the histogram sized %llu", s,
+                         (unsigned long long)pre.back(), (unsigned long long)expected);
+    struct Snap {  // the snapshot serves this scan only, whatever way it ends
+        mp_ctx *c;
+        ~Snap()
+        {
+            c->use_labsnap = false;
+            release(c, c->labsnap);
+        }
+    } snap{ctx};
+    // tuple buffers from earlier steps may be larger than a part needs (the
+    // histogram's staging): let each part size them
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (DevArray *a : {&ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
+                        &ctx->zbits, &ctx->runbits})
+        release(ctx, *a);
+    MP_TRY(snapshot_labels(ctx, s));
+    cap = std::min(cap, scan_tuple_room(ctx));  // the counts and the snapshot took their share
+    if (cap == 0) return ctx->fail(MP_ENOMEM, "the device budget leaves no room for a scan's tuples");
+    std::vector<uint64_t> cuts{lo};
+    while (cuts.back() < hi) {
+        const uint64_t a = cuts.back() - lo;
+        // last bucket end e with pre[e] - pre[a] <= cap
+        const uint64_t e = (uint64_t)(std::upper_bound(pre.begin() + a + 1, pre.end(), pre[a] + cap) - pre.begin()) - 1;
+        if (e == a)
+            return ctx->fail(MP_ENOMEM, "bucket %llu alone holds %llu tuples, more than the budget's %llu",
+                             (unsigned long long)(lo + a), (unsigned long long)(pre[a + 1] - pre[a]),
+                             (unsigned long long)cap);
+        cuts.push_back(lo + e);
+    }
+    const uint32_t mask = scan_pass_mask(ctx->cfg.k);
+    int passes = 0;
+    auto make_part = [&](uint64_t plo, uint64_t phi, uint64_t *n) -> int {
+        std::unique_ptr<GenSource> gen;
+        if (ctx->cfg.min3) {
+            MP_TRY(enumerate_scan(ctx, s, plo, phi, n));
+        } else {
+            gen.reset(new GenSource());
+            MP_TRY(gen_histograms(ctx, s, plo, phi, mask, false, gen.get(), n));
+        }
+        const uint64_t want = pre[phi - lo] - pre[plo - lo];
+        if (*n != want)
+            return ctx->fail(MP_ECOUNT, "scan %d part [%llu, %llu) made %llu tuples, counted %llu", s,
+                             (unsigned long long)plo, (unsigned long long)phi, (unsigned long long)*n,
+                             (unsigned long long)want);
+        return radix_sort_tuples(ctx, *n, &passes, gen.get());
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    MP_TRY(two_sweep_scan(ctx, cuts, seg_bounds, T, make_part, st));
+    st->sort_passes = passes;
+    st->ms_sort = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+                  st->ms_filter - st->ms_union;
+    return 0;
+}
+
 int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bounds, int T, uint64_t expected_tuples,
             mp_scan_stats *stats)
 {
@@ -247,6 +349,18 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
         MP_TRY(iota_u32(ctx, (uint32_t *)ctx->parent.p, N));
         MP_TRY(fill_u32(ctx, ctx->bad.p, 0, (N + 3) / 4));
         MP_TRY(fill_u32(ctx, ctx->filtered.p, 0, (N + 3) / 4));
+    }
+    // a scan whose tuples do not fit the budget goes twice over bucket parts that do
+    {
+        const uint64_t cap = scan_tuple_room(ctx);
+        if (expected_tuples && expected_tuples > cap) {
+            MP_TRY(scan_two_sweeps(ctx, s, lo, hi, seg_bounds, T, expected_tuples, cap, &st));
+            if (stats) *stats = st;
+            ctx->last_n = 0;
+            ctx->have_scan = false;  // only the last part's tuples are resident: nothing to dump
+            ctx->scans_done++;
+            return 0;
+        }
     }
     hipEventRecord(ctx->ev[0], ctx->stream);
     uint64_t n = 0;

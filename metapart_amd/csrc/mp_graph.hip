@@ -20,6 +20,8 @@
 // whatever order the lanes run in.
 #include "mp_internal.h"
 
+#include <functional>
+
 #include <algorithm>
 #include <chrono>
 #include <vector>
@@ -33,6 +35,7 @@ struct GParams {
     uint64_t n;
     const uint64_t *seg;  // T+1 tuple positions
     int T;
+    int open_last;        // the last segment goes on past these tuples (a part of a two-sweep scan)
     uint8_t *bad;
     uint8_t *filtered;
     uint32_t *zbits;
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         }
         bool zeroed = false;
         uint32_t v = 0;
-        if (e != s1) {  // the last run of a segment is never evaluated
+        if (e != s1 || (G.open_last && e == G.n)) {  // the last run of a segment is never evaluated
             ev++;
             if (e <= t0 + cnt_s) {
                 uint32_t in[8], out[8];
@@ -756,7 +759,7 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
     unsigned long long *d_stats = (unsigned long long *)(d_small + 8 * nseg);
     MP_TRY(fill_u32(ctx, d_stats, 0, 32));
     MP_HIP(hipMemcpyAsync(d_grp, grp, sizeof(uint64_t) * 2 * nseg, hipMemcpyHostToDevice, ctx->stream));
-    GParams G;
+    GParams G{};
     G.keys = keys;
     G.adj = adj;
     G.lab = lab;
@@ -865,6 +868,188 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
     const int rc = filter_union_core(ctx, n, d_seg, T, grp.data(), st);
     MP_HIP(hipFreeAsync(d_small, ctx->stream));
     return rc;
+}
+
+// ---------------------------------------------------------------- two sweeps
+// A scan whose tuples do not fit the device budget, over bucket parts that do
+// (cuts: part boundaries in bucket space, from lo to hi).  The filter's BAD set
+// must be complete before any union of the scan, so the parts go by twice:
+//   sweep 1: each part's tuples are made and sorted, and filtered (filtered
+//            marks, evaluated and zeroed runs); the scan's first label (tuple
+//            0's, from the first part) and last k-mer (from the last part, 0
+//            if zeroed) are kept for the first-run quirk; then the refilter
+//            turns the marks into the BAD set;
+//   sweep 2: each part's tuples are made, sorted and filtered again (for the
+//            zeroed runs and the run bitmaps; the marks repeat sweep 1's), the
+//            first surviving run of every segment not yet met takes the quirk,
+//            and the part's runs are joined.
+// Runs never cross a bucket, so never a part; a segment cut by a part end
+// keeps its last run evaluated there (open_last).  Joins in any order give the
+// same components, so the result is the one-pass scan's.
+int two_sweep_scan(mp_ctx *ctx, const std::vector<uint64_t> &cuts, const uint64_t *seg_bounds, int T,
+                   const std::function<int(uint64_t, uint64_t, uint64_t *)> &make_part, mp_scan_stats *st)
+{
+    const uint64_t N = ctx->N;
+    const int bshift = 2 * (ctx->cfg.k - 10);
+    const int P = (int)cuts.size() - 1;
+    uint64_t *d_small = nullptr;
+    const size_t small_words = 9 * (size_t)T + 2 * (size_t)(T + 1) + 32;
+    MP_HIP(hipMallocAsync((void **)&d_small, small_words * sizeof(uint64_t), ctx->stream));
+    uint64_t *d_first = d_small, *d_grp = d_small + T, *d_info = d_small + 3 * T, *d_bounds = d_small + 8 * T,
+             *d_seg = d_bounds + (T + 1);
+    unsigned long long *d_stats = (unsigned long long *)(d_seg + (T + 1));
+    MP_TRY(fill_u32(ctx, d_stats, 0, 2 * 32));
+    GParams G{};
+    G.bad = (uint8_t *)ctx->bad.p;
+    G.filtered = (uint8_t *)ctx->filtered.p;
+    G.parent = (uint32_t *)ctx->parent.p;
+    bool have_L0 = false, have_last = false;
+    uint32_t L0 = 0;
+    uint64_t lastK = 0, total = 0;
+    std::vector<char> met(T, 0);
+    int quirks = 0;
+    float ms_filter = 0, ms_union = 0, ms = 0;
+    // one part: its tuples sorted, its local segments, zeroed runs and run bitmaps
+    auto prepare = [&](int p, int sweep, uint64_t *n_out, int *nloc_out, std::vector<int> &segid) -> int {
+        const uint64_t plo = cuts[p], phi = cuts[p + 1];
+        uint64_t n = 0;
+        MP_TRY(make_part(plo, phi, &n));
+        *n_out = n;
+        segid.clear();
+        std::vector<uint64_t> lb;
+        for (int t = 0; t < T; t++) {
+            if (seg_bounds[t + 1] <= plo || seg_bounds[t] >= phi) continue;
+            lb.push_back(std::max(seg_bounds[t], plo));
+            segid.push_back(t);
+        }
+        lb.push_back(std::min(seg_bounds[segid.back() + 1], phi));
+        const int nloc = (int)segid.size();
+        *nloc_out = nloc;
+        if (n == 0) return 0;
+        MP_HIP(hipMemcpyAsync(d_bounds, lb.data(), sizeof(uint64_t) * (nloc + 1), hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(seg_pos_kernel, dim3(nblk(nloc + 1)), dim3(256), 0, ctx->stream,
+                           (const uint64_t *)ctx->keys[ctx->cur].p, n, bshift, (const uint64_t *)d_bounds, nloc, d_seg);
+        MP_HIP(hipGetLastError());
+        G.keys = (uint64_t *)ctx->keys[ctx->cur].p;
+        G.lab = (const uint32_t *)ctx->lab[ctx->cur].p;
+        G.adj = (const uint8_t *)ctx->adj[ctx->cur].p;
+        G.n = n;
+        G.seg = d_seg;
+        G.T = nloc;
+        G.open_last = seg_bounds[segid.back() + 1] > phi;
+        G.stats = d_stats + (sweep == 1 ? 0 : 16);  // the second sweep's counts repeat the first's
+        const uint64_t nwords = (n + 31) / 32 + 1, rwords = (n + 63) / 64 + 1;
+        MP_TRY(ensure(ctx, ctx->zbits, nwords * sizeof(uint32_t)));
+        MP_TRY(ensure(ctx, ctx->runbits, 2 * rwords * sizeof(uint64_t)));
+        G.zbits = (uint32_t *)ctx->zbits.p;
+        MP_TRY(fill_u32(ctx, ctx->zbits.p, 0, nwords));
+        if (!grid_fits((n + kFTile - 1) / kFTile, kFLanes))
+            return ctx->fail(MP_EINVAL, "filter over %llu tuples exceeds one dispatch", (unsigned long long)n);
+        hipEventRecord(ctx->ev[2], ctx->stream);
+        uint64_t *rs_bits = (uint64_t *)ctx->runbits.p;
+        hipLaunchKernelGGL(filter_tile_kernel, dim3((unsigned)((n + kFTile - 1) / kFTile)), dim3(kFLanes), 0,
+                           ctx->stream, G, rs_bits, rs_bits + rwords);
+        MP_HIP(hipGetLastError());
+        hipEventRecord(ctx->ev[3], ctx->stream);
+        // first survivor of every local segment and its k-mer and label; the part's
+        // first label and last k-mer (0 if zeroed)
+        std::vector<uint64_t> grp(2 * (size_t)nloc);
+        for (int l = 0; l < nloc; l++) {
+            grp[2 * l] = 0;
+            grp[2 * l + 1] = n;
+        }
+        MP_HIP(hipMemcpyAsync(d_grp, grp.data(), sizeof(uint64_t) * 2 * nloc, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(first_alive_kernel, dim3(nblk(nloc)), dim3(256), 0, ctx->stream, G, d_first);
+        hipLaunchKernelGGL(quirk_info_kernel, dim3(nblk(nloc)), dim3(256), 0, ctx->stream, G,
+                           (const uint64_t *)d_first, (const uint64_t *)d_grp, d_info);
+        MP_HIP(hipGetLastError());
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+        ms_filter += ms;
+        return 0;
+    };
+    std::vector<int> segid;
+    for (int p = 0; p < P; p++) {  // sweep 1
+        uint64_t n = 0;
+        int nloc = 0;
+        MP_TRY(prepare(p, 1, &n, &nloc, segid));
+        total += n;
+        if (!n) continue;
+        std::vector<uint64_t> info(5 * (size_t)nloc);
+        MP_HIP(hipMemcpy(info.data(), d_info, sizeof(uint64_t) * 5 * nloc, hipMemcpyDeviceToHost));
+        if (!have_L0) {
+            L0 = (uint32_t)info[4];
+            have_L0 = true;
+        }
+        lastK = info[3];
+        have_last = true;
+    }
+    hipEventRecord(ctx->ev[2], ctx->stream);
+    hipLaunchKernelGGL(refilter_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, G.bad,
+                       (const uint8_t *)G.filtered, G.parent, N, d_stats + 2);
+    MP_HIP(hipGetLastError());
+    hipEventRecord(ctx->ev[3], ctx->stream);
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    ms_filter += ms;
+    uint8_t bad_L0 = 0;
+    if (have_L0) MP_HIP(hipMemcpy(&bad_L0, G.bad + L0, 1, hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; p++) {  // sweep 2
+        uint64_t n = 0;
+        int nloc = 0;
+        MP_TRY(prepare(p, 2, &n, &nloc, segid));
+        if (!n) continue;
+        std::vector<uint64_t> info(5 * (size_t)nloc);
+        MP_HIP(hipMemcpy(info.data(), d_info, sizeof(uint64_t) * 5 * nloc, hipMemcpyDeviceToHost));
+        hipEventRecord(ctx->ev[4], ctx->stream);
+        for (int l = 0; l < nloc; l++) {  // metapart.c:2539-2616, as in the one-pass scan
+            const uint64_t *o = &info[5 * (size_t)l];
+            const int t = segid[l];
+            if (met[t] || o[0] == ~0ull) continue;
+            met[t] = 1;
+            if (!have_last || o[1] != lastK) continue;
+            if (bad_L0) {
+                MP_HIP(hipFreeAsync(d_small, ctx->stream));
+                return ctx->fail(MP_EPLAN, "first-run union quirk with a filtered tuple-0 label (segment %d)", t);
+            }
+            quirks++;
+            const uint32_t lf = (uint32_t)o[2];
+            if (lf != L0) {
+                hipLaunchKernelGGL(quirk_union_kernel, dim3(1), dim3(1), 0, ctx->stream, G.parent, L0, lf);
+                MP_HIP(hipGetLastError());
+            }
+        }
+        // the BAD set as bits (the sort of this part used the scratch array)
+        MP_TRY(ensure(ctx, ctx->scratch, ((N + 63) / 64 + 1) * sizeof(uint64_t)));
+        uint64_t *badbits = (uint64_t *)ctx->scratch.p;
+        hipLaunchKernelGGL(bad_bits_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint8_t *)G.bad, N,
+                           badbits);
+        const uint64_t rwords = (n + 63) / 64 + 1;
+        const uint64_t *rs_bits = (const uint64_t *)ctx->runbits.p;
+        hipLaunchKernelGGL(union_kernel, dim3(wave_grid((n + 63) / 64)), dim3(256), 0, ctx->stream, G, rs_bits,
+                           rs_bits + rwords, (const uint64_t *)badbits);
+        MP_HIP(hipGetLastError());
+        hipEventRecord(ctx->ev[5], ctx->stream);
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]);
+        ms_union += ms;
+    }
+    MP_TRY(compress_labels(ctx, G.parent, (const uint8_t *)G.bad, N));
+    unsigned long long hs[4] = {0, 0, 0, 0};
+    MP_HIP(hipMemcpyAsync(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipFreeAsync(d_small, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (st) {
+        st->tuples = total;
+        st->evaluated_runs = hs[0];
+        st->zeroed_runs = hs[1];
+        st->bad_labels = hs[2];
+        st->quirk_segments = quirks;
+        st->ms_filter = ms_filter;
+        st->ms_union = ms_union;
+        st->parts = P;
+    }
+    return 0;
 }
 
 int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int nseg, const uint64_t *grp,

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -157,6 +158,8 @@ struct mp_ctx {
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
+    mp::DevArray labsnap;  // a two-sweep scan's labels, taken at its start
+    bool use_labsnap = false;
     mp::DevArray khist;   // m-mer histogram: counts of the k-mer bytes the scans' radix passes read
     bool khist_valid = false;
     uint32_t khist_mask = 0;  // passes (bit 1 + byte) khist counts
@@ -249,6 +252,17 @@ int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T
 // per-segment quirk groups (2 per segment: first and end tuple, host).
 int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int nseg, const uint64_t *grp,
                         mp_scan_stats *st);
+// A scan over bucket parts (cuts[0..P], bucket boundaries) that together do
+// not fit the budget: filter sweep, then union sweep (see mp_graph.hip).
+// make_part(lo, hi, &n) leaves the part's sorted tuples in the current buffers.
+int two_sweep_scan(mp_ctx *ctx, const std::vector<uint64_t> &cuts, const uint64_t *seg_bounds, int T,
+                   const std::function<int(uint64_t, uint64_t, uint64_t *)> &make_part, mp_scan_stats *st);
+// Per-bucket k-mer counts of [lo, hi) (host prefix, hi - lo + 1 entries), by
+// device atomics: no staging memory beyond the counts.
+int bucket_prefix_range(mp_ctx *ctx, uint64_t lo, uint64_t hi, std::vector<uint64_t> &pre);
+// Labels of scan s for every read as they stand now; the scan's k-mer
+// kernels use them until use_labsnap is cleared.
+int snapshot_labels(mp_ctx *ctx, int s);
 int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st);
 int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t *W, const int64_t *il_start,
                      uint64_t *bytes_out);

@@ -32,6 +32,7 @@ struct KParams {
     uint32_t *lab;
     const uint32_t *parent;
     const uint8_t *bad;
+    const uint32_t *labels;  // when set: the labels of a scan taken at its start (a two-sweep scan)
     int scan0;
 };
 
@@ -100,6 +101,7 @@ __device__ __forceinline__ uint8_t adj_at(const KParams &P, uint64_t so, uint32_
 // read with its component root, or with itself once the filter marked it
 __device__ __forceinline__ uint32_t label_of(const KParams &P, uint64_t r)
 {
+    if (P.labels) return P.labels[r];
     return P.scan0 ? (uint32_t)r : (P.bad[r] ? (uint32_t)r : P.parent[r]);
 }
 

@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
     }
     if (r >= P.N || cnt == 0) return;
     uint64_t pos = P.wg_off[blockIdx.x] + woff + inc - cnt;
-    const uint32_t label = P.scan0 ? (uint32_t)r : (P.bad[r] ? (uint32_t)r : P.parent[r]);
+    const uint32_t label = label_of(P, r);
     min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, kmer, [&](uint64_t k, uint8_t a) {
         P.keys[pos] = k;
         P.adj[pos] = a;
@@ -530,6 +530,95 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
     }
 }
 
+// Per-bucket counts of the k-mers in [lo, hi) by device atomics (for a scan
+// that must be split into parts; no staging memory).
+__global__ __launch_bounds__(kKBlock) void kmer_bucket_count_kernel(KParams P, unsigned long long *__restrict__ cnt)
+{
+    __shared__ uint32_t s_pref[kRG + 1];
+    __shared__ uint64_t s_base[kRG];
+    const int tid = threadIdx.x;
+    const uint64_t ngrp = (P.N + kRG - 1) / kRG;
+    for (uint64_t g = blockIdx.x; g < ngrp; g += gridDim.x) {
+        const uint64_t r0 = g * kRG;
+        const int nr = (int)((P.N - r0) < (uint64_t)kRG ? (P.N - r0) : (uint64_t)kRG);
+        __syncthreads();  // the previous group is done with s_pref / s_base
+        const uint64_t p0 = P.pos_off[r0];
+        if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
+        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
+        __syncthreads();
+        const uint32_t total = s_pref[nr];
+        for (uint32_t q = tid; q < total; q += kKBlock) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pref[mid] <= q) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t p = q - s_pref[lo];
+            uint64_t canon;
+            bool fwd;
+            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
+            const uint64_t bk = canon >> P.bshift;
+            if (bk >= P.lo && bk < P.hi) atomicAdd(&cnt[bk - P.lo], 1ull);
+        }
+    }
+}
+
+__global__ void label_snap_kernel(const uint32_t *__restrict__ parent, const uint8_t *__restrict__ bad, uint64_t N,
+                                  int scan0, uint32_t *__restrict__ out)
+{
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) out[r] = scan0 ? (uint32_t)r : (bad[r] ? (uint32_t)r : parent[r]);
+}
+
+int snapshot_labels(mp_ctx *ctx, int s)
+{
+    MP_TRY(ensure(ctx, ctx->labsnap, ctx->N * sizeof(uint32_t) + 64));
+    hipLaunchKernelGGL(label_snap_kernel, dim3((unsigned)((ctx->N + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const uint32_t *)ctx->parent.p, (const uint8_t *)ctx->bad.p, ctx->N, s == 0 ? 1 : 0,
+                       (uint32_t *)ctx->labsnap.p);
+    MP_HIP(hipGetLastError());
+    ctx->use_labsnap = true;
+    return 0;
+}
+
+int bucket_prefix_range(mp_ctx *ctx, uint64_t lo, uint64_t hi, std::vector<uint64_t> &pre)
+{
+    const uint64_t nb = hi - lo;
+    pre.assign(nb + 1, 0);
+    if (ctx->cfg.min3) {  // the 3-minimizer counter adds per chunk with atomics already
+        const int C = ctx->C;
+        const uint64_t words = (uint64_t)C * (MP_NBUCKETS + 2);
+        MP_TRY(ensure(ctx, ctx->scratch, words * sizeof(uint32_t) + 4096));
+        MP_TRY(ensure(ctx, ctx->hist, (2 * (uint64_t)MP_NBUCKETS + 2) * sizeof(uint64_t)));
+        uint32_t *d_counts = (uint32_t *)ctx->scratch.p;
+        MP_TRY(count_chunks_device(ctx, ctx->chunk_ids.data(), C, d_counts));
+        uint64_t *d_tot = (uint64_t *)ctx->hist.p;
+        MP_TRY(bucket_totals(ctx, d_counts, C, d_tot));
+        std::vector<uint64_t> tot(MP_NBUCKETS);
+        MP_HIP(hipMemcpyAsync(tot.data(), d_tot, sizeof(uint64_t) * MP_NBUCKETS, hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint64_t b = 0; b < nb; b++) pre[b + 1] = pre[b] + (lo + b < MP_NBUCKETS ? tot[lo + b] : 0);
+        return 0;
+    }
+    KParams P = base_params(ctx);
+    P.lo = lo;
+    P.hi = hi;
+    MP_TRY(ensure(ctx, ctx->hist, (nb + 64) * sizeof(uint64_t)));
+    unsigned long long *d_cnt = (unsigned long long *)ctx->hist.p;
+    MP_TRY(fill_u32(ctx, d_cnt, 0, 2 * nb));
+    const uint64_t ngrp = (ctx->N + kRG - 1) / kRG;
+    const unsigned blocks = (unsigned)(ngrp < 8192 ? ngrp : 8192);
+    if (blocks) {
+        hipLaunchKernelGGL(kmer_bucket_count_kernel, dim3(blocks), dim3(kKBlock), 0, ctx->stream, P, d_cnt);
+        MP_HIP(hipGetLastError());
+    }
+    std::vector<uint64_t> c(nb);
+    MP_HIP(hipMemcpyAsync(c.data(), d_cnt, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint64_t b = 0; b < nb; b++) pre[b + 1] = pre[b] + c[b];
+    return 0;
+}
+
 int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_mask, bool all, GenSource *g,
                    uint64_t *n_out)
 {
@@ -540,6 +629,7 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
     P.scan0 = s == 0;
     P.parent = (const uint32_t *)ctx->parent.p;
     P.bad = (const uint8_t *)ctx->bad.p;
+    P.labels = ctx->use_labsnap ? (const uint32_t *)ctx->labsnap.p : nullptr;
     MP_HIP(hipMemcpyAsync(&g->npos, (const uint64_t *)ctx->pos_off.p + ctx->N, sizeof(uint64_t),
                           hipMemcpyDeviceToHost, ctx->stream));
     if (all && ctx->khist_valid && !(pass_mask & ~ctx->khist_mask)) {
@@ -585,6 +675,7 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
     P.scan0 = s == 0;
     P.parent = (const uint32_t *)ctx->parent.p;
     P.bad = (const uint8_t *)ctx->bad.p;
+    P.labels = ctx->use_labsnap ? (const uint32_t *)ctx->labsnap.p : nullptr;
     const uint64_t nwg = (ctx->N + kKBlock - 1) / kKBlock;
     MP_TRY(ensure(ctx, ctx->wg_cnt, nwg * sizeof(uint32_t)));
     MP_TRY(ensure(ctx, ctx->wg_off, (nwg + 1) * sizeof(uint64_t)));

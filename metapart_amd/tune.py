@@ -17,20 +17,21 @@ import numpy as np
 from . import NBUCKETS, Context, MPError, partition, plan_ranges, synth
 
 TUPLE_BYTES = 13           # k-mer u64 + label u32 + adj u8
-PER_TUPLE_DEVICE = 2 * TUPLE_BYTES + 1   # ping-pong buffers, sort status, filter/union bitmaps
-PER_READ_DEVICE = 70       # record table, labels, flags, sizes, pair-step arrays
-PER_READ_WRITER = 68       # two output items per record (sorted ping-pong) and their offsets
+PER_TUPLE_DEVICE = 29      # as the library plans a scan: ping-pong buffers, bitmaps, status, growth slack
+PER_READ_DEVICE = 74       # record table, labels, flags, sizes, a scan's label snapshot
+PER_READ_LATER = 128       # pair-step arrays and writer items, as the library keeps room for them
 TILE_SHAPES = (0, 8192, 4096, 6144, 3072)  # 0: 1024 lanes x 6 tuples
 
 
 def resident_bytes(reads: int, fq_bytes: int, bases: int, kmers: int) -> int:
-    """Device bytes that do not depend on the pass size."""
+    """Device bytes that do not depend on the pass size (the library's own
+    reckoning when it decides whether a scan fits, mp_api.hip)."""
     return (fq_bytes                      # FASTQ
             + PER_READ_DEVICE * reads     # per-read state
             + (3 * bases) // 8            # 2-bit + N streams
-            + PER_READ_WRITER * reads     # writer items
-            + fq_bytes                    # writer output streams (single-end: every record once)
-            + (64 << 20))
+            + bases // 16                 # look-back status of the generating pass
+            + (PER_READ_LATER * reads + fq_bytes) // 16 * 17  # pair step and writer (items, output streams)
+            + (512 << 20))                # histogram tables, counts per chunk, small arrays
 
 
 def scans_for_budget(hist: np.ndarray, reads: int, fq_bytes: int, bases: int, budget: int, T: int = 1,
