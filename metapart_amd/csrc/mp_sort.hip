@@ -211,7 +211,8 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
         const uint64_t fw0 = f0 >> 2, nf = (f1 + 3) / 4 - fw0;
         constexpr uint32_t kTab = 2 * 8 * kGenReads + 4 * (3 * kGenReads + 2);  // bytes of the read table
-        const bool staged = nr <= (uint32_t)kGenReads && kTab + 8 * (np + nn) + 4 * nf <= 8u * kTile;
+        const uint64_t mapo = (kTab + 8 * (np + nn) + 4 * nf + 7) & ~7ull;  // byte offset of the window map
+        const bool staged = nr <= (uint32_t)kGenReads && mapo + kTile <= 8u * kTile;
         uint64_t *s_rbase = s_buf, *s_rso = s_buf + kGenReads;
         uint32_t *s_rpos = (uint32_t *)(s_buf + 2 * kGenReads), *s_rlen = s_rpos + kGenReads + 1,
                  *s_rlab = s_rlen + kGenReads;
@@ -233,6 +234,16 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             for (uint32_t j = tid; j < nf; j += kSB) s_fw[j] = fq32[fw0 + j];
         }
         __syncthreads();
+        // window -> read of the tile (one byte each), so that the items need no search
+        uint8_t *s_map = (uint8_t *)s_buf + mapo;
+        const uint32_t q0 = (uint32_t)(base - pa);  // the tile's first window, relative to the first read
+        if (staged) {
+            for (uint32_t j = tid; j < nr; j += kSB) {
+                const uint32_t a = max(s_rpos[j], q0), e = min(s_rpos[j + 1], q0 + (uint32_t)kTile);
+                for (uint32_t w = a; w < e; w++) s_map[w - q0] = (uint8_t)j;
+            }
+            __syncthreads();
+        }
         auto rpos = [&](uint32_t j) -> uint32_t { return staged ? s_rpos[j] : (uint32_t)(A.G.pos_off[ra + j] - pa); };
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_nw[w - wn0] : A.G.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_pw[w - wp0] : A.G.packed[w]; };
@@ -249,13 +260,17 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             if (idx >= A.npos) continue;
             const uint32_t q = (uint32_t)(idx - pa);
             // the read holding window idx: last j with pos[j] <= q
-            for (int st = 0; st < 3 && lo + 1 < nr && rpos(lo + 1) <= q; st++) lo++;
-            if (lo + 1 < nr && rpos(lo + 1) <= q) {
-                uint32_t hi = nr - 1;
-                lo++;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if (rpos(mid) <= q) lo = mid; else hi = mid - 1;
+            if (staged) {
+                lo = s_map[q - q0];
+            } else {
+                for (int st = 0; st < 3 && lo + 1 < nr && rpos(lo + 1) <= q; st++) lo++;
+                if (lo + 1 < nr && rpos(lo + 1) <= q) {
+                    uint32_t hi = nr - 1;
+                    lo++;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (rpos(mid) <= q) lo = mid; else hi = mid - 1;
+                    }
                 }
             }
             const uint64_t r = ra + lo;
