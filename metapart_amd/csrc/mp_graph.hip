@@ -379,6 +379,8 @@ __device__ __forceinline__ bool is_bad(const uint64_t *bits, uint32_t x) { retur
 // word, or, for the first live tuple of a run that began in an earlier word,
 // by a walk back through that run.  Most tuples of this workload have BAD
 // labels, so the work is the label and BAD-bit reads, not the finds.
+constexpr int kUW = 4;  // bitmap words a wave takes per step, their loads in flight together
+
 __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *__restrict__ rs_bits,
                                                     const uint64_t *__restrict__ st_bits,
                                                     const uint64_t *__restrict__ badbits)
@@ -387,13 +389,20 @@ __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *_
     const unsigned lane = __lane_id();
     const uint64_t nwaves = (uint64_t)gridDim.x * blockDim.x / kWave;
     const uint64_t lt = lanemask_lt();
-    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; w < nw; w += nwaves) {
-        const uint64_t st = st_bits[w], rs = rs_bits[w];
-        // the run holding lane 0 when it began before this word: its start and whether it is joined
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave * kUW; w0 < nw;
+         w0 += nwaves * kUW) {
+        uint64_t st[kUW], rs[kUW];
+#pragma unroll
+        for (int u = 0; u < kUW; u++) {
+            st[u] = w0 + u < nw ? st_bits[w0 + u] : 0;
+            rs[u] = w0 + u < nw ? rs_bits[w0 + u] : 0;
+        }
+        // the run holding tuple w0 * 64 when it began before: its start and whether it is joined
         uint64_t cs = 0;
         bool cjoin = false;
-        if (!(st & 1) && w > 0) {
-            uint64_t q = w - 1, m = st_bits[q];
+        if (!(st[0] & 1) && w0 > 0) {
+            uint64_t q = w0 - 1, m = st_bits[q];
             while (!m && q > 0) m = st_bits[--q];
             if (m) {
                 const int bit = 63 - __builtin_clzll(m);
@@ -401,30 +410,59 @@ __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *_
                 cjoin = (rs_bits[q] >> bit) & 1;
             }
         }
-        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-        const uint64_t sm = st & upto;
-        const int sbit = sm ? 63 - __builtin_clzll(sm) : -1;  // start of the lane's run in this word
-        const bool join = sbit >= 0 ? ((rs >> sbit) & 1) : cjoin;
-        const uint64_t t = w * 64 + lane;
-        uint32_t b = 0;
-        bool live = false;
-        if (t < G.n && join) {
-            b = G.lab[t];
-            live = !is_bad(badbits, b);
+        uint32_t b[kUW];
+        bool live[kUW];
+        int sbit[kUW];
+        uint64_t ucs[kUW];  // start of the run holding the word's first tuple, for a run that began before it
+#pragma unroll
+        for (int u = 0; u < kUW; u++) {
+            if (u > 0 && !(st[u] & 1)) {  // carried from the words before in this step, or from before the step
+                bool found = false;
+#pragma unroll
+                for (int v = u - 1; v >= 0; v--) {
+                    if (!found && st[v]) {
+                        const int bit = 63 - __builtin_clzll(st[v]);
+                        ucs[u] = (w0 + v) * 64 + bit;
+                        found = true;
+                    }
+                }
+                if (!found) ucs[u] = cs;
+            } else {
+                ucs[u] = cs;
+            }
+            const uint64_t sm = st[u] & upto;
+            sbit[u] = sm ? 63 - __builtin_clzll(sm) : -1;  // start of the lane's run in this word
+            bool join;
+            if (sbit[u] >= 0) {
+                join = (rs[u] >> sbit[u]) & 1;
+            } else if (u == 0 || ucs[u] < w0 * 64) {
+                join = cjoin;
+            } else {
+                const uint64_t v = ucs[u] / 64 - w0;
+                join = (rs[v] >> (ucs[u] % 64)) & 1;
+            }
+            const uint64_t t = (w0 + u) * 64 + lane;
+            b[u] = t < G.n && join ? G.lab[t] : 0;
+            live[u] = t < G.n && join;
         }
-        uint64_t cand = __ballot(live) & lt;
-        if (sbit > 0) cand &= ~((1ull << sbit) - 1);  // lanes of the same run
-        const int p = cand ? 63 - __builtin_clzll(cand) : (int)lane;
-        const uint32_t a = __shfl(b, p, kWave);
-        if (!live) continue;
-        if (cand) {
-            if (a != b) uf_union(G.parent, a, b);
-        } else if (sbit < 0) {  // first live tuple of an earlier run in this word: the run's live tuple before it
-            for (uint64_t u = w * 64; u > cs;) {
-                const uint32_t c = G.lab[--u];
-                if (!is_bad(badbits, c)) {
-                    if (c != b) uf_union(G.parent, c, b);
-                    break;
+#pragma unroll
+        for (int u = 0; u < kUW; u++) live[u] = live[u] && !is_bad(badbits, b[u]);
+#pragma unroll
+        for (int u = 0; u < kUW; u++) {
+            uint64_t cand = __ballot(live[u]) & lt;
+            if (sbit[u] > 0) cand &= ~((1ull << sbit[u]) - 1);  // lanes of the same run
+            const int p = cand ? 63 - __builtin_clzll(cand) : (int)lane;
+            const uint32_t a = __shfl(b[u], p, kWave);
+            if (!live[u]) continue;
+            if (cand) {
+                if (a != b[u]) uf_union(G.parent, a, b[u]);
+            } else if (sbit[u] < 0) {  // first live tuple of an earlier run in this word: the run's live tuple before it
+                for (uint64_t x = (w0 + u) * 64; x > ucs[u];) {
+                    const uint32_t c = G.lab[--x];
+                    if (!is_bad(badbits, c)) {
+                        if (c != b[u]) uf_union(G.parent, c, b[u]);
+                        break;
+                    }
                 }
             }
         }
@@ -818,8 +856,8 @@ static int filter_union_core(mp_ctx *ctx, uint64_t n, const uint64_t *d_seg, int
         hipLaunchKernelGGL(bad_bits_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint8_t *)G.bad, N,
                            badbits);
         MP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(union_kernel, dim3(wave_grid((n + 63) / 64)), dim3(256), 0, ctx->stream, G,
-                           (const uint64_t *)rs_bits, (const uint64_t *)st_bits, (const uint64_t *)badbits);
+        hipLaunchKernelGGL(union_kernel, dim3(wave_grid(((n + 63) / 64 + kUW - 1) / kUW)), dim3(256), 0, ctx->stream,
+                           G, (const uint64_t *)rs_bits, (const uint64_t *)st_bits, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
     }
     MP_TRY(compress_labels(ctx, G.parent, (const uint8_t *)G.bad, N));
@@ -1026,8 +1064,8 @@ int two_sweep_scan(mp_ctx *ctx, const std::vector<uint64_t> &cuts, const uint64_
                            badbits);
         const uint64_t rwords = (n + 63) / 64 + 1;
         const uint64_t *rs_bits = (const uint64_t *)ctx->runbits.p;
-        hipLaunchKernelGGL(union_kernel, dim3(wave_grid((n + 63) / 64)), dim3(256), 0, ctx->stream, G, rs_bits,
-                           rs_bits + rwords, (const uint64_t *)badbits);
+        hipLaunchKernelGGL(union_kernel, dim3(wave_grid(((n + 63) / 64 + kUW - 1) / kUW)), dim3(256), 0,
+                           ctx->stream, G, rs_bits, rs_bits + rwords, (const uint64_t *)badbits);
         MP_HIP(hipGetLastError());
         hipEventRecord(ctx->ev[5], ctx->stream);
         MP_HIP(hipStreamSynchronize(ctx->stream));
