@@ -73,38 +73,6 @@ inline unsigned wave_grid(uint64_t n)
 // a dispatch holds fewer than 2^32 work-items (the packet's grid size is 32-bit)
 inline bool grid_fits(uint64_t blocks, uint64_t lanes) { return blocks * lanes < (1ull << 32); }
 
-// One wave copies len bytes from src to dst (any alignment): whole 32-bit
-// words where dst is aligned, each built from the two aligned source words it
-// straddles, and single bytes at the two ends.  Source words are read only
-// below src_end (the end of the source buffer); a word that would reach past
-// it is assembled from bytes.
-__device__ __forceinline__ void wave_copy(uint8_t *dst, const uint8_t *src, uint64_t len, const uint8_t *src_end,
-                                          unsigned lane)
-{
-    const uint32_t head = (uint32_t)((4 - ((uint64_t)dst & 3)) & 3);
-    const uint32_t h = head < len ? head : (uint32_t)len;
-    if (lane < h) dst[lane] = src[lane];
-    uint8_t *db = dst + h;
-    const uint8_t *sb = src + h;
-    const uint64_t rem = len - h, nw = rem / 4;
-    uint32_t *dw = (uint32_t *)db;
-    const uint32_t sh = (uint32_t)((uint64_t)sb & 3);
-    const uint32_t *sw = (const uint32_t *)((uint64_t)sb & ~3ull);
-    for (uint64_t m = lane; m < nw; m += kWave) {
-        uint32_t v;
-        if ((const uint8_t *)(sw + m + 2) <= src_end) {
-            const uint32_t a = sw[m];
-            v = sh ? (uint32_t)((((uint64_t)sw[m + 1] << 32) | a) >> (8 * sh)) : a;
-        } else {
-            const uint8_t *q = sb + 4 * m;
-            v = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
-        }
-        dw[m] = v;
-    }
-    const uint32_t t = (uint32_t)(rem & 3);
-    if (lane < t) db[4 * nw + lane] = sb[4 * nw + lane];
-}
-
 // Reverse-complement of a 2-bit packed k-mer (first base most significant,
 // A0 C1 G2 T3, complement = 3 - code): reverse the 2-bit groups of ~x.
 __device__ __forceinline__ uint64_t revcomp(uint64_t x, int K)

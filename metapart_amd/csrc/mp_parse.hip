@@ -10,6 +10,7 @@
 // charToIntMap[(c&6)>>1]) plus a 1-bit stream marking the exact byte 'N',
 // which is what the reference's lastNPos logic tests.
 #include "mp_internal.h"
+#include "mp_gather.h"
 
 namespace mp {
 
@@ -140,54 +141,13 @@ __global__ void rec_finalize_kernel(const uint8_t *__restrict__ fq, uint64_t n_r
     npos[r] = L > (uint64_t)K ? (uint32_t)(L - K + 1) : 0u;
 }
 
-// Packing in two coalesced steps: every read's sequence bytes are copied to a
-// dense base array (one wave per read, consecutive lanes on consecutive
-// bytes), then each lane turns 64 dense bases (four aligned 16-byte loads)
-// into two words of the 2-bit stream and one word of the N stream.
-__global__ __launch_bounds__(256) void dense_bases_kernel(const uint8_t *__restrict__ fq, uint64_t nbytes,
-                                                          const uint64_t *__restrict__ seq_off,
-                                                          const uint64_t *__restrict__ base_off, uint64_t n_rec,
-                                                          uint8_t *__restrict__ dense)
-{
-    const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / kWave;
-    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n_rec; r += waves) {
-        const uint64_t src = seq_off[r], dst = base_off[r], len = base_off[r + 1] - dst;
-        wave_copy(dense + dst, fq + src, len, fq + nbytes, __lane_id());
-    }
-}
-
-__global__ __launch_bounds__(256) void pack_dense_kernel(const uint8_t *__restrict__ dense, uint64_t total_bases,
-                                                         uint64_t n_words, uint64_t *__restrict__ packed,
-                                                         uint64_t *__restrict__ nmask)
-{
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_words) return;
-    uint64_t w0 = 0, w1 = 0, nm = 0;
-    const uint64_t b0 = t * 64;
-    if (b0 < total_bases) {
-        const uint4 *v = (const uint4 *)(dense + b0);
-        const uint32_t nb = (uint32_t)min((uint64_t)64, total_bases - b0);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint4 x = v[q];
-            const uint32_t wd[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int j = q * 16 + e;
-                const uint8_t c = (uint8_t)(wd[e >> 2] >> (8 * (e & 3)));
-                if ((uint32_t)j < nb) {
-                    const uint64_t code = (uint64_t)((0x2310u >> (((c & 6) >> 1) * 4)) & 3);  // {0,1,3,2}[(c&6)>>1]
-                    if (j < 32) w0 |= code << (62 - 2 * j);
-                    else w1 |= code << (62 - 2 * (j - 32));
-                    if (c == 'N') nm |= 1ull << (63 - j);
-                }
-            }
-        }
-    }
-    packed[2 * t] = w0;
-    packed[2 * t + 1] = w1;
-    nmask[t] = nm;
-}
+// Packing: the sequence bytes of the reads are gathered in base order (read
+// r's bases at base_off[r]) and turned into the 2-bit and 'N' streams in the
+// same pass, 16 bases per lane (mp_gather.h).
+struct SeqSrc {
+    const uint64_t *seq_off;
+    __device__ uint64_t operator()(uint64_t r) const { return seq_off[r]; }
+};
 
 // chunk_ofs[c] -> index of the record starting there (or n_rec for the end);
 // flags a chunk boundary that is not a record start.
@@ -307,17 +267,14 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     const uint64_t nthreads = (tb + 63) / 64 + 2;  // two zero words of padding after the stream
     MP_TRY(ensure(ctx, ctx->packed, nthreads * 2 * sizeof(uint64_t)));
     MP_TRY(ensure(ctx, ctx->nmask, nthreads * sizeof(uint64_t)));
-    // the dense bases stage in the second key buffer, unused until the first sort
-    MP_TRY(ensure(ctx, ctx->keys[1], nthreads * 64 + 64));
-    uint8_t *dense = (uint8_t *)ctx->keys[1].p;
-    if (N) {
-        hipLaunchKernelGGL(dense_bases_kernel, dim3(wave_grid(N)), dim3(256), 0, ctx->stream, fq, ctx->fq_bytes,
-                           (const uint64_t *)ctx->seq_off.p, (const uint64_t *)ctx->base_off.p, N, dense);
-        MP_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(pack_dense_kernel, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const uint8_t *)dense, tb, nthreads, (uint64_t *)ctx->packed.p, (uint64_t *)ctx->nmask.p);
-    MP_HIP(hipGetLastError());
+    uint64_t *first = nullptr;
+    const uint64_t nb = (tb + kGBlock - 1) / kGBlock;
+    MP_HIP(hipMallocAsync((void **)&first, (nb + 1) * sizeof(uint64_t), ctx->stream));
+    const int rc = launch_gather(ctx, fq, ctx->fq_bytes, (const uint64_t *)ctx->base_off.p, N, tb, nthreads * 64, first,
+                                 SeqSrc{(const uint64_t *)ctx->seq_off.p},
+                                 PackSink{(uint32_t *)ctx->packed.p, (uint16_t *)ctx->nmask.p});
+    MP_HIP(hipFreeAsync(first, ctx->stream));
+    MP_TRY(rc);
     MP_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "mp_internal.h"
+#include "mp_gather.h"
 
 namespace mp {
 
@@ -113,21 +114,12 @@ __global__ void gather_len_kernel(const uint32_t *__restrict__ slot, const uint3
     if (i < n) out[i] = len[slot[i]];
 }
 
-// one wave per item (grid-stride over items): lanes copy consecutive bytes
-__global__ void copy_items_kernel(const uint8_t *__restrict__ fq, uint64_t nbytes, const uint64_t *__restrict__ rec_off,
-                                  const uint32_t *__restrict__ slot, const uint64_t *__restrict__ off, uint64_t n_items,
-                                  uint8_t *__restrict__ out)
-{
-    const unsigned lane = __lane_id();
-    const uint64_t waves = (uint64_t)gridDim.x * blockDim.x / kWave;
-    for (uint64_t item = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; item < n_items; item += waves) {
-        const uint64_t r = slot[item] >> 1;
-        const uint64_t src = rec_off[r];
-        const uint64_t dst = off[item];
-        const uint64_t len = off[item + 1] - dst;
-        wave_copy(out + dst, fq + src, len, fq + nbytes, lane);
-    }
-}
+// source of item i (in file order): its record, or the first record of its pair
+struct RecSrc {
+    const uint64_t *rec_off;
+    const uint32_t *slot;
+    __device__ uint64_t operator()(uint64_t i) const { return rec_off[slot[i] >> 1]; }
+};
 
 // first item of every file (and the end), and the byte offset there
 __global__ void file_bounds_kernel(const uint64_t *__restrict__ file, uint64_t n, int F, const uint64_t *__restrict__ off,
@@ -246,11 +238,14 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     MP_HIP(hipStreamSynchronize(ctx->stream));
     MP_TRY(ensure(ctx, ctx->out, total + 64));
     const uint64_t real_items = fpos[F];
-    if (real_items) {
-        hipLaunchKernelGGL(copy_items_kernel, dim3(wave_grid(real_items)), dim3(256), 0, ctx->stream,
-                           ctx->d_fq, ctx->fq_bytes, (const uint64_t *)ctx->rec_off.p, slot, (const uint64_t *)off, real_items,
-                           (uint8_t *)ctx->out.p);
-        MP_HIP(hipGetLastError());
+    if (real_items && total) {
+        uint64_t *first = nullptr;
+        MP_HIP(hipMallocAsync((void **)&first, ((total + kGBlock - 1) / kGBlock + 1) * sizeof(uint64_t), ctx->stream));
+        const int rc = launch_gather(ctx, ctx->d_fq, ctx->fq_bytes, (const uint64_t *)off, real_items, total, total,
+                                     first, RecSrc{(const uint64_t *)ctx->rec_off.p, slot},
+                                     CopySink{(uint8_t *)ctx->out.p});
+        MP_HIP(hipFreeAsync(first, ctx->stream));
+        MP_TRY(rc);
     }
     hipEventRecord(ctx->ev[1], ctx->stream);
     MP_HIP(hipFreeAsync(d_small, ctx->stream));
