@@ -77,6 +77,48 @@ __device__ __forceinline__ bool kmer_from(int K, uint64_t b, uint32_t p, NWord n
     return true;
 }
 
+// A window rolled along a read: forward code F and reverse complement R of
+// the last K bases, and the run of bases without 'N' ending at the last one.
+// init() builds the window at base b whole; roll() appends base bn (the base
+// after the window) and drops the first.  canon() applies the K = 32 quirk of
+// kmer_from (forward code 0 past a read's first position).
+struct RollKmer {
+    uint64_t F, R;
+    uint32_t nrun;
+    template <typename NWord, typename PWord>
+    __device__ __forceinline__ void init(int K, uint64_t b, NWord nword, PWord pword)
+    {
+        const int o = (int)(b & 63);
+        const uint64_t nh = nword(b >> 6), nl = nword((b >> 6) + 1);
+        const int sn = o + K;
+        uint64_t v = sn <= 64 ? (nh >> (64 - sn)) : ((nh << (sn - 64)) | (nl >> (128 - sn)));
+        v &= K < 64 ? (1ull << K) - 1 : ~0ull;
+        nrun = v ? (uint32_t)__builtin_ctzll(v) : (uint32_t)K;  // bases after the window's last 'N'
+        const int op = (int)(b & 31);
+        const uint64_t ph = pword(b >> 5), pl = pword((b >> 5) + 1);
+        const int s = 2 * (op + K);
+        F = s <= 64 ? (ph >> (64 - s)) : ((ph << (s - 64)) | (pl >> (128 - s)));
+        if (K < 32) F &= (1ull << (2 * K)) - 1;
+        R = revcomp(F, K);
+    }
+    template <typename NWord, typename PWord>
+    __device__ __forceinline__ void roll(int K, uint64_t bn, NWord nword, PWord pword)
+    {
+        const uint64_t c = (pword(bn >> 5) >> (62 - 2 * (bn & 31))) & 3;
+        const uint64_t nb = (nword(bn >> 6) >> (63 - (bn & 63))) & 1;
+        F = (F << 2) | c;
+        if (K < 32) F &= (1ull << (2 * K)) - 1;
+        R = (R >> 2) | ((3 - c) << (2 * K - 2));
+        nrun = nb ? 0u : nrun + 1;
+    }
+    __device__ __forceinline__ bool valid(int K) const { return nrun >= (uint32_t)K; }
+    __device__ __forceinline__ uint64_t canon(int K, uint32_t p) const
+    {
+        const uint64_t f = (K == 32 && p > 0) ? 0 : F;
+        return f < R ? f : R;
+    }
+};
+
 __device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
 {
     return kmer_from(P.K, b, p, [&](uint64_t w) { return P.nmask[w]; }, [&](uint64_t w) { return P.packed[w]; },

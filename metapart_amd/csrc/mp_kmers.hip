@@ -118,25 +118,44 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
+        // each lane takes a contiguous run of the group's windows and rolls the
+        // forward and reverse codes from one window to the next (one base in,
+        // one out); a read's first window is built whole (kmer_from)
         const uint32_t total = s_pref[nr];
-        for (uint32_t q = tid; q < total; q += kKBlock) {
+        const uint32_t cs = (total + kKBlock - 1) / kKBlock;
+        uint32_t q = tid * cs;
+        const uint32_t qe = min(q + cs, total);
+        if (q < qe) {
             int lo = 0, h = nr - 1;
             while (lo < h) {
                 const int mid = (lo + h + 1) >> 1;
                 if (s_pref[mid] <= q) lo = mid; else h = mid - 1;
             }
-            const uint32_t p = q - s_pref[lo];
-            uint64_t canon;
-            bool fwd;
-            if (!kmer_from(P.K, s_base[lo] + p, p, nword, pword, canon, fwd)) continue;
-            const uint32_t b = (uint32_t)(canon >> P.bshift);
-            const uint32_t d = b >> kHLowBits;
-            const uint32_t slot = atomicAdd(&s_dig[d], 1u);
-            if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
-            if (PHASE == 0) {  // digit counts of a later sort over every k-mer
+            uint32_t p = q - s_pref[lo], pend = s_pref[lo + 1] - s_pref[lo];
+            RollKmer k;
+            k.init(P.K, s_base[lo] + p, nword, pword);
+            for (;;) {
+                if (k.valid(P.K)) {
+                    const uint64_t canon = k.canon(P.K, p);
+                    const uint32_t b = (uint32_t)(canon >> P.bshift);
+                    const uint32_t d = b >> kHLowBits;
+                    const uint32_t slot = atomicAdd(&s_dig[d], 1u);
+                    if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
+                    if (PHASE == 0) {  // digit counts of a later sort over every k-mer
 #pragma unroll
-                for (int y = 0; y < 8; y++)
-                    if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
+                        for (int y = 0; y < 8; y++)
+                            if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
+                    }
+                }
+                if (++q >= qe) break;
+                if (++p < pend) {
+                    k.roll(P.K, s_base[lo] + p + P.K - 1, nword, pword);
+                } else {  // the next read with a window (q < total: there is one)
+                    do lo++; while (s_pref[lo + 1] == s_pref[lo]);
+                    p = 0;
+                    pend = s_pref[lo + 1] - s_pref[lo];
+                    k.init(P.K, s_base[lo], nword, pword);
+                }
             }
         }
     }
