@@ -119,23 +119,44 @@ struct RollKmer {
     }
 };
 
-__device__ __forceinline__ bool kmer_at(const KParams &P, uint64_t b, uint32_t p, uint64_t &canon, bool &fwd)
+// Calls fn(canonical k-mer) for every window without 'N' of a group of nr
+// reads (window prefix pref[0..nr], first bases base[0..nr)): each of the
+// nlanes lanes takes a contiguous run of the group's windows and rolls the
+// k-mer from one window to the next, building a read's first window whole.
+template <typename NWord, typename PWord, typename Fn>
+__device__ __forceinline__ void for_group_windows(int K, const uint32_t *pref, const uint64_t *base, int nr,
+                                                  uint32_t lane_id, uint32_t nlanes, NWord nword, PWord pword, Fn fn)
 {
-    return kmer_from(P.K, b, p, [&](uint64_t w) { return P.nmask[w]; }, [&](uint64_t w) { return P.packed[w]; },
-                     canon, fwd);
+    const uint32_t total = pref[nr];
+    const uint32_t cs = (total + nlanes - 1) / nlanes;
+    uint32_t q = lane_id * cs;
+    const uint32_t qe = min(q + cs, total);
+    if (q >= qe) return;
+    int lo = 0, h = nr - 1;  // the read of q: last lo with pref[lo] <= q
+    while (lo < h) {
+        const int mid = (lo + h + 1) >> 1;
+        if (pref[mid] <= q) lo = mid; else h = mid - 1;
+    }
+    uint32_t p = q - pref[lo], pend = pref[lo + 1] - pref[lo];
+    RollKmer k;
+    k.init(K, base[lo] + p, nword, pword);
+    for (;;) {
+        if (k.valid(K)) fn(k.canon(K, p));
+        if (++q >= qe) break;
+        if (++p < pend) {
+            k.roll(K, base[lo] + p + K - 1, nword, pword);
+        } else {  // the next read with a window (q < total: there is one)
+            do lo++; while (pref[lo + 1] == pref[lo]);
+            p = 0;
+            pend = pref[lo + 1] - pref[lo];
+            k.init(K, base[lo], nword, pword);
+        }
+    }
 }
 
 // adj byte from the bytes before and after the window (enumerateKmers.c:215, 239)
 __device__ __forceinline__ uint8_t adj_from(uint8_t prev, uint8_t next, bool fwd)
 {
-    return fwd ? adj_of(prev, next) : adj_of(rc_char(next), rc_char(prev));
-}
-
-// adj byte of the window at p of a read (sequence at fq[so], length L)
-__device__ __forceinline__ uint8_t adj_at(const KParams &P, uint64_t so, uint32_t L, uint32_t p, bool fwd)
-{
-    const uint8_t prev = p == 0 ? (uint8_t)'N' : P.fq[so + p - 1];
-    const uint8_t next = p == L - (uint32_t)P.K ? (uint8_t)'N' : P.fq[so + p + P.K];
     return fwd ? adj_of(prev, next) : adj_of(rc_char(next), rc_char(prev));
 }
 

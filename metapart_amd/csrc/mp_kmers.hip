@@ -3,14 +3,16 @@
 //
 // Plain variant (every k-mer; countMmers.c:83-217, enumerateKmers.c:127-371).
 // The reference rolls a forward and a reverse-complement code through each
-// read on one CPU thread.  Here the lanes of a workgroup walk the flattened
-// list of their reads' k-mer positions, so every lane does O(1) work per
-// k-mer regardless of read length (kmer_at):
+// read on one CPU thread.  Here the lanes of a workgroup split the flattened
+// list of a group of reads' k-mer positions into equal contiguous runs, so
+// the work per lane does not depend on read length (for_group_windows):
 //   F  = 2K bits of the packed stream at the k-mer's first base (two 64-bit
 //        loads and a funnel shift; MSB-first packing makes this the
-//        reference's curKmerInt),
-//   R  = reverse complement of F by bit reversal (the reference's curRcInt),
-//   N  = the same window of the 1-bit 'N' stream must be empty (lastNPos),
+//        reference's curKmerInt) for a lane's first window of a read, then
+//        rolled one base at a time, as the reference does,
+//   R  = reverse complement of F (bit reversal, then rolled: the reference's
+//        curRcInt),
+//   N  = bases since the last 'N' of the 1-bit 'N' stream must reach K (lastNPos),
 //   canonical = F < R ? F : R, bucket = canonical >> 2(K-10) (getMmerPrefix).
 // At K = 32 the reference's mask (1 << 64) - 1 evaluates to 0 on x86, so every
 // forward code after the first position is 0; that is reproduced.
@@ -118,46 +120,17 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
-        // each lane takes a contiguous run of the group's windows and rolls the
-        // forward and reverse codes from one window to the next (one base in,
-        // one out); a read's first window is built whole (kmer_from)
-        const uint32_t total = s_pref[nr];
-        const uint32_t cs = (total + kKBlock - 1) / kKBlock;
-        uint32_t q = tid * cs;
-        const uint32_t qe = min(q + cs, total);
-        if (q < qe) {
-            int lo = 0, h = nr - 1;
-            while (lo < h) {
-                const int mid = (lo + h + 1) >> 1;
-                if (s_pref[mid] <= q) lo = mid; else h = mid - 1;
-            }
-            uint32_t p = q - s_pref[lo], pend = s_pref[lo + 1] - s_pref[lo];
-            RollKmer k;
-            k.init(P.K, s_base[lo] + p, nword, pword);
-            for (;;) {
-                if (k.valid(P.K)) {
-                    const uint64_t canon = k.canon(P.K, p);
-                    const uint32_t b = (uint32_t)(canon >> P.bshift);
-                    const uint32_t d = b >> kHLowBits;
-                    const uint32_t slot = atomicAdd(&s_dig[d], 1u);
-                    if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
-                    if (PHASE == 0) {  // digit counts of a later sort over every k-mer
+        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon) {
+            const uint32_t b = (uint32_t)(canon >> P.bshift);
+            const uint32_t d = b >> kHLowBits;
+            const uint32_t slot = atomicAdd(&s_dig[d], 1u);
+            if (PHASE == 1) H.items[s_out[d] + slot] = (uint16_t)(b & (kHLow - 1));
+            if (PHASE == 0) {  // digit counts of a later sort over every k-mer
 #pragma unroll
-                        for (int y = 0; y < 8; y++)
-                            if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
-                    }
-                }
-                if (++q >= qe) break;
-                if (++p < pend) {
-                    k.roll(P.K, s_base[lo] + p + P.K - 1, nword, pword);
-                } else {  // the next read with a window (q < total: there is one)
-                    do lo++; while (s_pref[lo + 1] == s_pref[lo]);
-                    p = 0;
-                    pend = s_pref[lo + 1] - s_pref[lo];
-                    k.init(P.K, s_base[lo], nword, pword);
-                }
+                for (int y = 0; y < 8; y++)
+                    if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
             }
-        }
+        });
     }
     if (PHASE == 0) {
         __syncthreads();
@@ -287,10 +260,10 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
 
 // ---------------------------------------------------------------- 3-minimizer
 // The reference rolls forward and reverse codes through the read
-// (countMmers_3minimizers.c:107-190); position i's canonical k-mer and its
-// 'N' test are exactly kmer_at(), which reads the packed streams (8x smaller
-// than the FASTQ bytes, so a lane walking its read stays in cache).  The
-// window state machine itself is sequential per read and is restated as is.
+// (countMmers_3minimizers.c:107-190); so does the lane of a read here
+// (RollKmer over the packed streams, 8x smaller than the FASTQ bytes, so a
+// lane walking its read stays in cache).  The window state machine itself is
+// sequential per read and is restated as is.
 
 // counter: countMmers_3minimizers.c:83-271 (no range filter)
 template <typename Src, typename Sink>
@@ -396,7 +369,26 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
         L = (int)P.seq_len[r];
         b0 = P.base_off[r];
     }
-    auto kmer = [&](int i, uint64_t &c, bool &fwd) { return kmer_at(P, b0 + (uint64_t)i, (uint32_t)i, c, fwd); };
+    // the window logic asks for positions 0, 1, 2, ... in order: roll the k-mer
+    RollKmer rk;
+    int cur = -1;
+    auto nword = [&](uint64_t w) { return P.nmask[w]; };
+    auto pword = [&](uint64_t w) { return P.packed[w]; };
+    auto kmer = [&](int i, uint64_t &c, bool &fwd) {
+        if (cur < 0 || i < cur) {
+            rk.init(P.K, b0 + (uint64_t)i, nword, pword);
+            cur = i;
+        }
+        while (cur < i) {
+            cur++;
+            rk.roll(P.K, b0 + (uint64_t)cur + P.K - 1, nword, pword);
+        }
+        if (!rk.valid(P.K)) return false;
+        const uint64_t f = (P.K == 32 && i > 0) ? 0 : rk.F;
+        fwd = f < rk.R;
+        c = fwd ? f : rk.R;
+        return true;
+    };
     if (MODE == kModeHist) {
         if (r < P.N) {
             int lo = 0, hi = P.C - 1;
@@ -524,23 +516,13 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
-        const uint32_t total = s_pref[nr];
-        for (uint32_t q = tid; q < total; q += kKBlock) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pref[mid] <= q) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t p = q - s_pref[lo];
-            uint64_t canon;
-            bool fwd;
-            if (!kmer_from(P.K, s_base[lo] + p, p, nword, pword, canon, fwd)) continue;
+        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon) {
             const uint64_t bk = canon >> P.bshift;
-            if (bk < P.lo || bk >= P.hi) continue;
+            if (bk < P.lo || bk >= P.hi) return;
 #pragma unroll
             for (int b = 0; b < 8; b++)
                 if ((pass_mask >> (b + 1)) & 1) atomicAdd(&h[b][(canon >> (8 * b)) & 0xFF], 1u);
-        }
+        });
     }
     __syncthreads();
     for (int j = tid; j < 8 * kSortRadix; j += kKBlock) {
@@ -565,20 +547,13 @@ __global__ __launch_bounds__(kKBlock) void kmer_bucket_count_kernel(KParams P, u
         if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
         if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
         __syncthreads();
-        const uint32_t total = s_pref[nr];
-        for (uint32_t q = tid; q < total; q += kKBlock) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pref[mid] <= q) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t p = q - s_pref[lo];
-            uint64_t canon;
-            bool fwd;
-            if (!kmer_at(P, s_base[lo] + p, p, canon, fwd)) continue;
-            const uint64_t bk = canon >> P.bshift;
-            if (bk >= P.lo && bk < P.hi) atomicAdd(&cnt[bk - P.lo], 1ull);
-        }
+        for_group_windows(
+            P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) { return P.nmask[w]; },
+            [&](uint64_t w) { return P.packed[w]; },
+            [&](uint64_t canon) {
+                const uint64_t bk = canon >> P.bshift;
+                if (bk >= P.lo && bk < P.hi) atomicAdd(&cnt[bk - P.lo], 1ull);
+            });
     }
 }
 
