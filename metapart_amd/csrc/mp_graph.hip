@@ -531,10 +531,26 @@ __global__ void compress_kernel(uint32_t *parent, const uint8_t *__restrict__ ba
     parent[i] = x;
 }
 
+// sizes[r] += reads x with parent[x] = r.  A component holding much of the
+// input (the 3-minimizer graph before its pair step) would otherwise
+// serialise millions of device atomics on one address, so the lanes of a wave
+// that share the root of its lowest lane left add once, for up to eight such
+// roots; the rest add singly (one atomic each, as when every root differs).
+constexpr int kSizeRounds = 8;
 __global__ void sizes_kernel(const uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes, uint64_t N)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) atomicAdd(&sizes[parent[i]], 1u);
+    bool act = i < N;
+    const uint32_t v = act ? parent[i] : 0u;
+    int round = 0;
+    for (uint64_t left = __ballot(act); left && round < kSizeRounds; left = __ballot(act), round++) {
+        const int lead = __builtin_ctzll(left);
+        const uint32_t lv = __shfl(v, lead, kWave);
+        const uint64_t same = __ballot(act && v == lv);
+        if ((int)__lane_id() == lead) atomicAdd(&sizes[lv], (uint32_t)__popcll(same));
+        if (v == lv) act = false;
+    }
+    if (act) atomicAdd(&sizes[v], 1u);
 }
 
 __global__ void root_weight_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
