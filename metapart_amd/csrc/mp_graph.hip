@@ -23,7 +23,9 @@
 #include <functional>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 namespace mp {
@@ -640,14 +642,43 @@ __global__ void big_root_kernel(const uint32_t *__restrict__ parent, const uint3
     if (x < N) flag[x] = parent[x] == (uint32_t)x && gsize[gparent[x]] > MP_PAIR_GATE;
 }
 
-// ordered pairs, in pair order, as (compact id of read 2j's root, of read 2j+1's root)
+// roots of the candidate-graph components of > 1000 reads (numbered by a scan)
+__global__ void big_groot_kernel(const uint32_t *__restrict__ gparent, const uint32_t *__restrict__ gsize,
+                                 uint32_t *__restrict__ flag, uint64_t N)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < N) flag[x] = gparent[x] == (uint32_t)x && gsize[x] > MP_PAIR_GATE;
+}
+
+// ordered pairs, in pair order, as (compact id of read 2j's root, of read 2j+1's root),
+// with the number of their candidate-graph component and their position
 __global__ void pair_list_kernel(const uint32_t *__restrict__ ordered, const uint64_t *__restrict__ off, uint64_t P,
                                  const uint32_t *__restrict__ parent, const uint64_t *__restrict__ cid,
-                                 uint64_t *__restrict__ pairs)
+                                 const uint32_t *__restrict__ gparent, const uint64_t *__restrict__ gid,
+                                 uint64_t *__restrict__ pairs, uint64_t *__restrict__ comp, uint32_t *__restrict__ pos)
 {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < P && ordered[j])
-        pairs[off[j]] = (uint64_t)(uint32_t)cid[parent[2 * j]] | ((uint64_t)(uint32_t)cid[parent[2 * j + 1]] << 32);
+    if (j < P && ordered[j]) {
+        const uint64_t q = off[j];
+        const uint32_t a = parent[2 * j];
+        pairs[q] = (uint64_t)(uint32_t)cid[a] | ((uint64_t)(uint32_t)cid[parent[2 * j + 1]] << 32);
+        comp[q] = gid[gparent[a]];
+        pos[q] = (uint32_t)q;
+    }
+}
+
+// the ordered pairs grouped by component (pair order kept inside one), and
+// each component's range [cf, cl) of them
+__global__ void pair_group_kernel(const uint64_t *__restrict__ comp, const uint32_t *__restrict__ pos,
+                                  const uint64_t *__restrict__ pairs, uint64_t n, uint64_t *__restrict__ out,
+                                  uint32_t *__restrict__ cf, uint32_t *__restrict__ cl)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = pairs[pos[i]];
+    const uint64_t g = comp[i];
+    if (i == 0 || comp[i - 1] != g) cf[g] = (uint32_t)i;
+    if (i + 1 == n || comp[i + 1] != g) cl[g] = (uint32_t)(i + 1);
 }
 
 __global__ void compact_roots_kernel(const uint32_t *__restrict__ flag, const uint64_t *__restrict__ cid,
@@ -1165,8 +1196,10 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     uint64_t n_ordered = 0;
     std::vector<uint64_t> hpairs;
     std::vector<uint32_t> cs;
-    uint64_t M = 0;  // roots in candidate-graph components of > 1000 reads
-    uint32_t *d_flag = nullptr, *d_csize = nullptr, *d_corig = nullptr, *d_cfinal = nullptr;
+    uint64_t M = 0;   // roots in candidate-graph components of > 1000 reads
+    uint64_t Kg = 0;  // those components
+    std::vector<uint32_t> cfl;  // their ranges of ordered pairs: first [0, Kg), ends [Kg, 2 Kg)
+    uint32_t *d_flag = nullptr, *d_csize = nullptr, *d_corig = nullptr, *d_cfinal = nullptr, *d_cfl = nullptr;
     uint64_t *d_cid = nullptr, *d_pairs = nullptr;
     if (P) {
         MP_HIP(hipMemcpyAsync(gparent, parent, N * 4, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1184,31 +1217,60 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
                               ctx->stream));
         MP_HIP(hipStreamSynchronize(ctx->stream));
         if (n_ordered) {  // the ordered pairs on compact root ids, and those roots' sizes
-            MP_TRY(ensure(ctx, ctx->pairc, N * 4 + (N + 1) * 8 + n_ordered * 8 + 3 * N * 4 + 256));
+            MP_TRY(ensure(ctx, ctx->pairc, N * 4 + 2 * (N + 1) * 8 + n_ordered * 8 + 3 * N * 4 + 256));
             uint8_t *pc = (uint8_t *)ctx->pairc.p;
             d_flag = (uint32_t *)pc;
             d_cid = (uint64_t *)(pc + ((N * 4 + 7) & ~7ull));
-            d_pairs = d_cid + (N + 1);
+            uint64_t *d_gid = d_cid + (N + 1);
+            d_pairs = d_gid + (N + 1);
             d_csize = (uint32_t *)(d_pairs + n_ordered);
             d_corig = d_csize + N;
             d_cfinal = d_corig + N;
+            // pairs of different candidate-graph components share no roots: they
+            // are grouped by component (a stable sort), and the components run
+            // on several host threads
+            for (int b = 0; b < 2; b++) {
+                MP_TRY(ensure(ctx, ctx->items[b], n_ordered * sizeof(uint64_t)));
+                MP_TRY(ensure(ctx, ctx->ilens[b], n_ordered * sizeof(uint32_t)));
+                MP_TRY(ensure(ctx, ctx->iaux[b], n_ordered + 16));
+            }
+            hipLaunchKernelGGL(big_groot_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)gparent,
+                               (const uint32_t *)gsize, d_flag, N);
+            MP_HIP(hipGetLastError());
+            MP_TRY(scan_u32(ctx, d_flag, d_gid, N));
             hipLaunchKernelGGL(big_root_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
                                (const uint32_t *)gparent, (const uint32_t *)gsize, d_flag, N);
             MP_HIP(hipGetLastError());
             MP_TRY(scan_u32(ctx, d_flag, d_cid, N));
             hipLaunchKernelGGL(pair_list_kernel, dim3(nblk(P)), dim3(256), 0, ctx->stream, (const uint32_t *)ordered,
                                (const uint64_t *)ctx->wg_off.p, P, (const uint32_t *)parent, (const uint64_t *)d_cid,
-                               d_pairs);
+                               (const uint32_t *)gparent, (const uint64_t *)d_gid, d_pairs,
+                               (uint64_t *)ctx->items[0].p, (uint32_t *)ctx->ilens[0].p);
             hipLaunchKernelGGL(compact_roots_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream,
                                (const uint32_t *)d_flag, (const uint64_t *)d_cid, (const uint32_t *)sizes, d_csize,
                                d_corig, N);
             MP_HIP(hipGetLastError());
             MP_HIP(hipMemcpyAsync(&M, d_cid + N, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(&Kg, d_gid + N, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
             MP_HIP(hipStreamSynchronize(ctx->stream));
+            const uint32_t gmask = Kg <= 256 ? 0x2u : Kg <= 65536 ? 0x6u : Kg <= (1u << 24) ? 0xEu : 0x1Eu;
+            int cur = 0, gpasses = 0;
+            MP_TRY(radix_sort_generic(ctx, ctx->items, ctx->ilens, ctx->iaux, &cur, n_ordered, gmask, &gpasses,
+                                      nullptr, 0, nullptr));
+            MP_HIP(hipMallocAsync((void **)&d_cfl, 2 * Kg * sizeof(uint32_t), ctx->stream));
+            MP_TRY(fill_u32(ctx, d_cfl, 0, 2 * Kg));
+            uint64_t *d_sorted = (uint64_t *)ctx->items[1 - cur].p;
+            hipLaunchKernelGGL(pair_group_kernel, dim3(nblk(n_ordered)), dim3(256), 0, ctx->stream,
+                               (const uint64_t *)ctx->items[cur].p, (const uint32_t *)ctx->ilens[cur].p,
+                               (const uint64_t *)d_pairs, n_ordered, d_sorted, d_cfl, d_cfl + Kg);
+            MP_HIP(hipGetLastError());
             hpairs.resize(n_ordered);
             cs.resize(M);
-            MP_HIP(hipMemcpyAsync(hpairs.data(), d_pairs, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
+            cfl.resize(2 * Kg);
+            MP_HIP(hipMemcpyAsync(hpairs.data(), d_sorted, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
             MP_HIP(hipMemcpyAsync(cs.data(), d_csize, M * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(cfl.data(), d_cfl, 2 * Kg * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipFreeAsync(d_cfl, ctx->stream));
         }
         hipLaunchKernelGGL(pair_apply_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, parent,
                            (const uint32_t *)gparent, (const uint32_t *)gsize, N, d_unions);
@@ -1221,18 +1283,38 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<uint32_t> cp(M);
         for (uint64_t c = 0; c < M; c++) cp[c] = (uint32_t)c;
-        for (uint64_t q = 0; q < n_ordered; q++) {
-            const uint32_t a = hfind(cp.data(), (uint32_t)hpairs[q]), b = hfind(cp.data(), (uint32_t)(hpairs[q] >> 32));
-            if (a == b) continue;
-            if (cs[a] <= MP_PAIR_GATE || cs[b] <= MP_PAIR_GATE) {
-                unions++;
-                if (a > b) { cp[b] = a; cs[a] += cs[b]; cs[b] = 0; }
-                else { cp[a] = b; cs[b] += cs[a]; cs[a] = 0; }
+        // each component's pairs in pair order, the reference's loop; the
+        // components touch disjoint entries of cp and cs
+        std::atomic<uint64_t> next{0};
+        const unsigned nth = (unsigned)std::max<uint64_t>(
+            1, std::min<uint64_t>({16, std::max(1u, std::thread::hardware_concurrency()), Kg}));
+        std::vector<uint64_t> tun(nth, 0);
+        auto work = [&](unsigned t) {
+            for (uint64_t g; (g = next.fetch_add(1)) < Kg;) {
+                for (uint64_t q = cfl[g]; q < cfl[Kg + g]; q++) {
+                    const uint32_t a = hfind(cp.data(), (uint32_t)hpairs[q]),
+                                   b = hfind(cp.data(), (uint32_t)(hpairs[q] >> 32));
+                    if (a == b) continue;
+                    if (cs[a] <= MP_PAIR_GATE || cs[b] <= MP_PAIR_GATE) {
+                        tun[t]++;
+                        if (a > b) { cp[b] = a; cs[a] += cs[b]; cs[b] = 0; }
+                        else { cp[a] = b; cs[b] += cs[a]; cs[a] = 0; }
+                    }
+                }
             }
+        };
+        std::vector<std::thread> pool;
+        try {
+            for (unsigned t = 1; t < nth; t++) pool.emplace_back(work, t);
+        } catch (const std::exception &) {  // fewer threads: the others take their components
         }
-        for (uint64_t c = 0; c < M; c++) cp[c] = hfind(cp.data(), (uint32_t)c);
+        work(0);
+        for (auto &th : pool) th.join();
+        for (unsigned t = 0; t < nth; t++) unions += tun[t];
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        // the final root of every compact root, by pointer jumping on the device
         MP_HIP(hipMemcpyAsync(d_cfinal, cp.data(), M * 4, hipMemcpyHostToDevice, ctx->stream));
+        MP_TRY(compress_labels(ctx, d_cfinal, nullptr, M));
         hipLaunchKernelGGL(pair_merge_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
                            (const uint32_t *)gparent, (const uint32_t *)gsize, (const uint64_t *)d_cid,
                            (const uint32_t *)d_cfinal, (const uint32_t *)d_corig, N);
