@@ -1151,6 +1151,8 @@ int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int ns
 // ---------------------------------------------------------------- finish
 // Sequential pair step (unionFindPeReads, metapart.c:2801-2867) on the host,
 // for the pairs whose outcome depends on the order.
+constexpr uint64_t kPairAhead = 16;  // pairs between a prefetch and its use
+
 static uint32_t hfind(uint32_t *parent, uint32_t x)
 {
     while (parent[x] != x) {
@@ -1291,7 +1293,13 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         std::vector<uint64_t> tun(nth, 0);
         auto work = [&](unsigned t) {
             for (uint64_t g; (g = next.fetch_add(1)) < Kg;) {
-                for (uint64_t q = cfl[g]; q < cfl[Kg + g]; q++) {
+                const uint64_t qb = cfl[g], qe = cfl[Kg + g];
+                for (uint64_t q = qb; q < qe; q++) {
+                    if (q + kPairAhead < qe) {  // the entries a later pair starts from
+                        const uint64_t f = hpairs[q + kPairAhead];
+                        __builtin_prefetch(&cp[(uint32_t)f]);
+                        __builtin_prefetch(&cp[(uint32_t)(f >> 32)]);
+                    }
                     const uint32_t a = hfind(cp.data(), (uint32_t)hpairs[q]),
                                    b = hfind(cp.data(), (uint32_t)(hpairs[q] >> 32));
                     if (a == b) continue;
