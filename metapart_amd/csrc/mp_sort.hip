@@ -617,13 +617,16 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
 // high bits a group holds a few tuples.  The local pass gives each tuple its
 // rank inside its group by (k-mer, position) and writes it there: the same
 // array the low-byte passes would have produced, for the cost of one pass.
-// A workgroup owns the groups that start in its 2048-tuple window (staged in
-// LDS with up to 2048 more for the last group); a longer group goes to a list
-// and is sorted by its low bytes with the global passes on its own range.
+// A workgroup owns the groups that start in its 1024-tuple window: the window
+// and the next 256 positions are loaded into LDS at once, group starts are an
+// LDS bitmap, and the window's last group may take up to 1024 positions past
+// the window; a longer group goes to a list and is sorted by its low bytes
+// with the global passes on its own range.
 #ifndef MP_LOCAL_WIN
 #define MP_LOCAL_WIN 1024
 #endif
 constexpr int kLWin = MP_LOCAL_WIN, kLMax = MP_LOCAL_WIN, kLCap = kLWin + kLMax, kLThreads = 256;
+constexpr int kLMarg = 256;  // positions loaded past the window for the next group start
 
 struct LocalPass {
     const uint64_t *keys_in;
@@ -645,51 +648,77 @@ __device__ __forceinline__ bool group_start(const uint64_t *keys, uint64_t i, in
 
 __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
 {
+    // keys of [w0, w0 + kLCap) relative to w0, then the order keys of the groups
     __shared__ uint64_t s_key[kLCap];
-    __shared__ uint64_t s_st[kLCap / kWave];
-    __shared__ unsigned long long s_r0, s_r1;
+    __shared__ uint64_t s_st[kLCap / kWave];  // group starts, bit j % 64 of word j / 64 for w0 + j
+    __shared__ unsigned long long s_r1;
+    __shared__ uint64_t s_prev;
     const uint64_t w0 = (uint64_t)blockIdx.x * kLWin, w1 = min(w0 + kLWin, L.n);
     const uint64_t lim = min(w1 + kLMax, L.n);
-    if (threadIdx.x == 0) {
-        s_r0 = w1;   // first group start in [w0, w1)
-        s_r1 = lim;  // first group start in [w1, lim), or lim when none (then the last group is long)
-    }
-    __syncthreads();
     const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // first group start at or after w0 (below w1) and after w1 (below lim):
-    // 256 positions per step, one atomic per wave, stop at the first step with one
-    for (uint64_t c0 = w0; c0 < w1; c0 += kLThreads) {
-        const uint64_t i = c0 + threadIdx.x;
-        const bool st = i < w1 && group_start(L.keys_in, i, L.shift);
-        const uint64_t m = __ballot(st);
-        if (lane == 0 && m) atomicMin(&s_r0, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
-        if (__syncthreads_or(st)) break;
-    }
-    for (uint64_t c0 = w1; c0 < lim; c0 += kLThreads) {
-        const uint64_t i = c0 + threadIdx.x;
-        const bool st = i < lim && group_start(L.keys_in, i, L.shift);
-        const uint64_t m = __ballot(st);
-        if (lane == 0 && m) atomicMin(&s_r1, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
-        if (__syncthreads_or(st)) break;
+    // the window and a margin after it, in one coalesced load; group starts
+    // from LDS (the first position of the array is a start)
+    const uint32_t nld = (uint32_t)(min(w1 + kLMarg, lim) - w0);
+    for (uint32_t j = threadIdx.x; j < nld; j += kLThreads) s_key[j] = L.keys_in[w0 + j];
+    if (threadIdx.x == 0) {
+        s_prev = w0 ? L.keys_in[w0 - 1] : 0;
+        s_r1 = lim;
     }
     __syncthreads();
-    const uint64_t r0 = s_r0;
-    uint64_t r1 = s_r1;
-    if (r0 >= w1) return;  // no group starts here
-    if (r1 == lim && lim < L.n) {  // the last group of the window runs past w1 + kLMax: hand it over
-        __shared__ unsigned long long s_last;
-        if (threadIdx.x == 0) s_last = r0;
-        __syncthreads();
-        for (uint64_t c0 = r0; c0 < w1; c0 += kLThreads) {
+    for (uint32_t j0 = wave * kWave; j0 < nld; j0 += kLThreads) {
+        const uint32_t j = j0 + lane;
+        const bool st = j < nld && (w0 + j == 0 || (s_key[j] >> L.shift) != ((j ? s_key[j - 1] : s_prev) >> L.shift));
+        const uint64_t m = __ballot(st);
+        if (lane == 0) s_st[j0 / kWave] = m;
+    }
+    __syncthreads();
+    const uint32_t e0 = (uint32_t)(w1 - w0), nwl = (nld + kWave - 1) / kWave;
+    // first start in [w0, w1): none means no group starts here
+    uint32_t r0 = e0;
+    for (uint32_t w = 0; w * kWave < e0; w++)
+        if (s_st[w]) {
+            r0 = w * kWave + (uint32_t)__builtin_ctzll(s_st[w]);
+            break;
+        }
+    if (r0 >= e0) return;  // uniform: every lane read the same words
+    // first start in [w1, lim), from the margin, else from memory (a group
+    // that runs past the margin); lim when there is none
+    uint32_t r1 = nld;
+    for (uint32_t w = e0 / kWave; w < nwl; w++) {
+        uint64_t b = s_st[w];
+        if (w == e0 / kWave) b &= ~0ull << (e0 % kWave);
+        if (b) {
+            r1 = w * kWave + (uint32_t)__builtin_ctzll(b);
+            break;
+        }
+    }
+    if (r1 >= nld && w0 + nld < lim) {
+        for (uint64_t c0 = w0 + nld; c0 < lim; c0 += kLThreads) {
             const uint64_t i = c0 + threadIdx.x;
-            const uint64_t m = __ballot(i < w1 && group_start(L.keys_in, i, L.shift));
-            if (lane == 0 && m) atomicMax(&s_last, (unsigned long long)(c0 + wave * kWave + 63 - (uint64_t)__builtin_clzll(m)));
+            const bool st = i < lim && group_start(L.keys_in, i, L.shift);
+            const uint64_t m = __ballot(st);
+            if (lane == 0 && m) atomicMin(&s_r1, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
+            if (__syncthreads_or(st)) break;
         }
         __syncthreads();
-        r1 = s_last;
-        if (threadIdx.x == 0) L.ovf[atomicAdd(L.novf, 1ull)] = s_last;
+        r1 = (uint32_t)(s_r1 - w0);
+        if (s_r1 == lim && lim < L.n) {  // the last group of the window runs past w1 + kLMax: hand it over
+            uint32_t last = r0;
+            for (uint32_t w = 0; w * kWave < e0; w++) {
+                uint64_t b = s_st[w];
+                if ((w + 1) * kWave > e0) b &= e0 % kWave ? (1ull << (e0 % kWave)) - 1 : ~0ull;
+                if (b) last = w * kWave + 63 - (uint32_t)__builtin_clzll(b);
+            }
+            r1 = last;
+            if (threadIdx.x == 0) L.ovf[atomicAdd(L.novf, 1ull)] = w0 + last;
+        } else {  // the rest of the window's last group, and no start inside it
+            for (uint32_t j = nld + threadIdx.x; j < r1; j += kLThreads) s_key[j] = L.keys_in[w0 + j];
+            for (uint32_t w = nwl + threadIdx.x; w * kWave < r1; w += kLThreads) s_st[w] = 0;
+            __syncthreads();
+        }
     }
-    const uint32_t m = (uint32_t)(r1 - r0);
+    // groups [r0, r1): each tuple's rank inside its group by (low bits, position)
+    const uint32_t m = r1 - r0;
     constexpr int kQ = kLCap / kLThreads;
     uint64_t kr[kQ], c[kQ];  // keys, and (low bits << 12) | index: the order within a group, ties by position
     const uint64_t lowmask = (1ull << L.shift) - 1;
@@ -697,48 +726,37 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
         if (j < m) {
-            kr[q] = L.keys_in[r0 + j];
-            s_key[j] = kr[q];
+            kr[q] = s_key[r0 + j];
             c[q] = ((kr[q] & lowmask) << 12) | j;
         }
     }
     __syncthreads();
-    // group starts as a bitmap: word w holds positions 64w .. 64w+63 (one wave's positions)
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
-        if (q * kLThreads >= m) break;
-        const bool st = j < m && (j == 0 || (s_key[j] >> L.shift) != (s_key[j - 1] >> L.shift));
-        const uint64_t b = __ballot(st);
-        if (lane == 0) s_st[j / kWave] = b;
+        if (j < m) s_key[r0 + j] = c[q];
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < kQ; q++) {
-        const uint32_t j = q * kLThreads + threadIdx.x;
-        if (j < m) s_key[j] = c[q];
-    }
-    __syncthreads();
-    const uint32_t nw = (m + kWave - 1) / kWave;
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
         if (q * kLThreads >= m) break;
         if (j >= m) continue;
-        uint32_t w = j / kWave;
-        const uint32_t bit = j % kWave;
+        const uint32_t p = r0 + j;  // position relative to w0
+        uint32_t w = p / kWave;
+        const uint32_t bit = p % kWave;
         const uint64_t upto = bit == 63 ? ~0ull : (2ull << bit) - 1;
-        uint64_t bs = s_st[w] & upto;  // position 0 is a start: the search ends
+        uint64_t bs = s_st[w] & upto;  // r0 is a start: the search ends
         while (!bs) bs = s_st[--w];
         const uint32_t gs = w * kWave + 63 - (uint32_t)__builtin_clzll(bs);
-        w = j / kWave;
+        w = p / kWave;
         uint64_t be = s_st[w] & ~upto;
-        while (!be && ++w < nw) be = s_st[w];
-        const uint32_t ge = be ? w * kWave + (uint32_t)__builtin_ctzll(be) : m;
+        while (!be && ++w * kWave < r1) be = s_st[w];
+        const uint32_t ge = be ? min(r1, w * kWave + (uint32_t)__builtin_ctzll(be)) : r1;
         const uint64_t cj = c[q];
         uint32_t rank = gs;
         for (uint32_t t = gs; t < ge; t++) rank += s_key[t] < cj;
-        const uint64_t src = r0 + j, dst = r0 + rank;
+        const uint64_t src = w0 + p, dst = w0 + rank;
         L.keys_out[dst] = kr[q];
         L.lab_out[dst] = L.lab_in[src];
         L.adj_out[dst] = L.adj_in[src];
