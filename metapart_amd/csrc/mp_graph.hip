@@ -534,53 +534,70 @@ __global__ void compress_kernel(uint32_t *parent, const uint8_t *__restrict__ ba
 }
 
 // sizes[r] += reads x with parent[x] = r.  A component holding much of the
-// input (the 3-minimizer graph before its pair step) would otherwise
-// serialise millions of device atomics on one address, so the lanes of a wave
-// that share the root of its lowest lane left add once, for up to eight such
-// roots; the rest add singly (one atomic each, as when every root differs).
-constexpr int kSizeRounds = 8;
-__global__ void sizes_kernel(const uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes, uint64_t N)
+// input (the 3-minimizer graph before its pair step, a 100 M-read scan's
+// largest component) would serialise millions of device atomics on one
+// address, so a workgroup first counts its 256 reads' roots in a small LDS
+// hash table (four probes; a root that finds no slot adds singly) and then
+// adds each root it saw once.
+constexpr uint32_t kSizeSlots = 512;
+__global__ __launch_bounds__(256) void sizes_kernel(const uint32_t *__restrict__ parent, uint32_t *__restrict__ sizes,
+                                                    uint64_t N)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = i < N;
-    const uint32_t v = act ? parent[i] : 0u;
-    int round = 0;
-    for (uint64_t left = __ballot(act); left && round < kSizeRounds; left = __ballot(act), round++) {
-        const int lead = __builtin_ctzll(left);
-        const uint32_t lv = __shfl(v, lead, kWave);
-        const uint64_t same = __ballot(act && v == lv);
-        if ((int)__lane_id() == lead) atomicAdd(&sizes[lv], (uint32_t)__popcll(same));
-        if (v == lv) act = false;
+    __shared__ uint32_t s_key[kSizeSlots], s_cnt[kSizeSlots];
+    for (uint32_t j = threadIdx.x; j < kSizeSlots; j += blockDim.x) {
+        s_key[j] = ~0u;
+        s_cnt[j] = 0;
     }
-    if (act) atomicAdd(&sizes[v], 1u);
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        const uint32_t v = parent[i];
+        const uint32_t h = (v * 2654435761u) >> 23;  // 9 bits
+        bool done = false;
+        for (uint32_t p = 0; p < 4 && !done; p++) {
+            const uint32_t slot = (h + p) & (kSizeSlots - 1);
+            const uint32_t old = atomicCAS(&s_key[slot], ~0u, v);
+            if (old == ~0u || old == v) {
+                atomicAdd(&s_cnt[slot], 1u);
+                done = true;
+            }
+        }
+        if (!done) atomicAdd(&sizes[v], 1u);
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < kSizeSlots; j += blockDim.x)
+        if (s_key[j] != ~0u) atomicAdd(&sizes[s_key[j]], s_cnt[j]);
 }
 
-__global__ void root_weight_kernel(const uint32_t *__restrict__ parent, const uint32_t *__restrict__ sizes,
-                                   uint32_t *__restrict__ w, uint64_t N, unsigned long long *__restrict__ st)
+__global__ __launch_bounds__(256) void root_weight_kernel(const uint32_t *__restrict__ parent,
+                                                          const uint32_t *__restrict__ sizes, uint32_t *__restrict__ w,
+                                                          uint64_t N, unsigned long long *__restrict__ st)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool root = false;
-    uint32_t s = 0;
-    if (i < N) {
-        root = parent[i] == (uint32_t)i;
-        s = root ? sizes[i] : 0;
+    // grid-stride; then workgroup totals: number of roots and the largest size, one atomic each
+    uint32_t nroot = 0, mx = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        const bool root = parent[i] == (uint32_t)i;
+        const uint32_t s = root ? sizes[i] : 0;
         w[i] = s;
+        nroot += root;
+        mx = max(mx, s);
     }
-    // block totals: number of roots and the largest size, one atomic each
     __shared__ uint32_t s_n[256 / kWave], s_m[256 / kWave];
-    const uint64_t m = __ballot(root);
-    uint32_t mx = s;
-    for (int d = kWave / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_down(mx, d, kWave));
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        nroot += __shfl_down(nroot, d, kWave);
+        mx = max(mx, (uint32_t)__shfl_down(mx, d, kWave));
+    }
     if (__lane_id() == 0) {
-        s_n[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+        s_n[threadIdx.x / kWave] = nroot;
         s_m[threadIdx.x / kWave] = mx;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0, b = 0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); w++) {
-            a += s_n[w];
-            b = max(b, s_m[w]);
+        for (int k = 0; k < (int)(blockDim.x / kWave); k++) {
+            a += s_n[k];
+            b = max(b, s_m[k]);
         }
         if (a) atomicAdd(&st[0], (unsigned long long)a);
         if (b) atomicMax(&st[1], (unsigned long long)b);
@@ -1338,7 +1355,7 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     // partition plan over root ids in ascending order
     MP_TRY(ensure(ctx, ctx->wg_off, (N + 1) * sizeof(uint64_t)));
     MP_TRY(ensure(ctx, ctx->wg_cnt, N * sizeof(uint32_t)));
-    hipLaunchKernelGGL(root_weight_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
+    hipLaunchKernelGGL(root_weight_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, (const uint32_t *)parent,
                        (const uint32_t *)sizes, (uint32_t *)ctx->wg_cnt.p, N, (unsigned long long *)(d_small + 32));
     MP_HIP(hipGetLastError());
     MP_TRY(scan_u32(ctx, (const uint32_t *)ctx->wg_cnt.p, (uint64_t *)ctx->wg_off.p, N));
