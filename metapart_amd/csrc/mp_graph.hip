@@ -1170,11 +1170,15 @@ int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int ns
 // for the pairs whose outcome depends on the order.
 constexpr uint64_t kPairAhead = 16;  // pairs between a prefetch and its use
 
-static uint32_t hfind(uint32_t *parent, uint32_t x)
+struct PairNode {  // a compact root: its parent, and its component's size while it is a root
+    uint32_t p, s;
+};
+
+static uint32_t hfind(PairNode *nd, uint32_t x)
 {
-    while (parent[x] != x) {
-        const uint32_t p = parent[x], gp = parent[p];
-        parent[x] = gp;
+    while (nd[x].p != x) {
+        const uint32_t p = nd[x].p, gp = nd[p].p;
+        nd[x].p = gp;
         x = gp;
     }
     return x;
@@ -1300,10 +1304,11 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     if (n_ordered) {
         MP_HIP(hipStreamSynchronize(ctx->stream));
         const auto t0 = std::chrono::steady_clock::now();
-        std::vector<uint32_t> cp(M);
-        for (uint64_t c = 0; c < M; c++) cp[c] = (uint32_t)c;
+        // parent and size side by side: a find ends on the line it reads the size from
+        std::vector<PairNode> nd(M);
+        for (uint64_t c = 0; c < M; c++) nd[c] = {(uint32_t)c, cs[c]};
         // each component's pairs in pair order, the reference's loop; the
-        // components touch disjoint entries of cp and cs
+        // components touch disjoint nodes
         std::atomic<uint64_t> next{0};
         const unsigned nth = (unsigned)std::max<uint64_t>(
             1, std::min<uint64_t>({16, std::max(1u, std::thread::hardware_concurrency()), Kg}));
@@ -1312,18 +1317,18 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
             for (uint64_t g; (g = next.fetch_add(1)) < Kg;) {
                 const uint64_t qb = cfl[g], qe = cfl[Kg + g];
                 for (uint64_t q = qb; q < qe; q++) {
-                    if (q + kPairAhead < qe) {  // the entries a later pair starts from
+                    if (q + kPairAhead < qe) {  // the nodes a later pair starts from
                         const uint64_t f = hpairs[q + kPairAhead];
-                        __builtin_prefetch(&cp[(uint32_t)f]);
-                        __builtin_prefetch(&cp[(uint32_t)(f >> 32)]);
+                        __builtin_prefetch(&nd[(uint32_t)f]);
+                        __builtin_prefetch(&nd[(uint32_t)(f >> 32)]);
                     }
-                    const uint32_t a = hfind(cp.data(), (uint32_t)hpairs[q]),
-                                   b = hfind(cp.data(), (uint32_t)(hpairs[q] >> 32));
+                    const uint32_t a = hfind(nd.data(), (uint32_t)hpairs[q]),
+                                   b = hfind(nd.data(), (uint32_t)(hpairs[q] >> 32));
                     if (a == b) continue;
-                    if (cs[a] <= MP_PAIR_GATE || cs[b] <= MP_PAIR_GATE) {
+                    if (nd[a].s <= MP_PAIR_GATE || nd[b].s <= MP_PAIR_GATE) {
                         tun[t]++;
-                        if (a > b) { cp[b] = a; cs[a] += cs[b]; cs[b] = 0; }
-                        else { cp[a] = b; cs[b] += cs[a]; cs[a] = 0; }
+                        if (a > b) { nd[b].p = a; nd[a].s += nd[b].s; nd[b].s = 0; }
+                        else { nd[a].p = b; nd[b].s += nd[a].s; nd[a].s = 0; }
                     }
                 }
             }
@@ -1338,6 +1343,8 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         for (unsigned t = 0; t < nth; t++) unions += tun[t];
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         // the final root of every compact root, by pointer jumping on the device
+        std::vector<uint32_t> cp(M);
+        for (uint64_t c = 0; c < M; c++) cp[c] = nd[c].p;
         MP_HIP(hipMemcpyAsync(d_cfinal, cp.data(), M * 4, hipMemcpyHostToDevice, ctx->stream));
         MP_TRY(compress_labels(ctx, d_cfinal, nullptr, M));
         hipLaunchKernelGGL(pair_merge_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
