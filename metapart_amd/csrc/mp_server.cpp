@@ -5,7 +5,8 @@
 // engines, each an mp_ctx with its own stream and its own share of the HBM
 // budget, run on host threads.  An engine that goes idle takes, in arrival
 // order, every queued job whose estimated device footprint still fits its
-// share (at most max_jobs_per_batch) and partitions them as one batch
+// share (at most max_jobs_per_batch and 256 MiB of FASTQ) and partitions
+// them as one batch
 // (mp_batch_load / mp_batch_partition); jobs that arrive meanwhile wait for the
 // next idle engine, so the work of two batches overlaps on the device and new
 // jobs are admitted as engines and memory free up.  A batch that fails is
@@ -45,6 +46,12 @@ struct Job {
 // k-mer tuple per base (bases <= bytes / 2) at 26 B for the ping-pong tuple
 // buffers plus sort, filter and writer state.
 uint64_t estimate_bytes(uint64_t nbytes) { return nbytes + nbytes + (nbytes / 2) * 32 + (8u << 20); }
+
+// FASTQ bytes of one batch (a larger job runs alone): about 800 k reads of
+// 150 bp, a few ms of device work, so the jobs of a batch, which all finish
+// with it, do not wait on a long one; batching pays off for the small jobs,
+// whose fixed per-sample costs it shares.
+constexpr uint64_t kBatchBytes = 256ull << 20;
 
 }  // namespace
 
@@ -127,11 +134,12 @@ void mp_server::engine_loop(int e)
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             if (queue.empty()) return;  // stopping, nothing left
-            uint64_t used = 0;
+            uint64_t used = 0, bytes = 0;
             while (!queue.empty() && (int)batch.size() < cfg.max_jobs_per_batch) {
-                const uint64_t est = queue.front()->est;
-                if (!batch.empty() && used + est > share) break;
-                used += est;
+                const Job &j = *queue.front();
+                if (!batch.empty() && (used + j.est > share || bytes + j.nbytes > kBatchBytes)) break;
+                used += j.est;
+                bytes += j.nbytes;
                 batch.push_back(queue.front());
                 queue.pop_front();
             }

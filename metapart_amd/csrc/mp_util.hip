@@ -9,6 +9,11 @@ int ensure(mp_ctx *ctx, DevArray &a, uint64_t bytes)
     if (bytes == 0) bytes = 256;
     if (a.bytes >= bytes) return 0;
     uint64_t want = bytes + (bytes >> 4);  // 6 % slack so slowly growing runs do not reallocate each time
+    // a buffer that grows again at least doubles while a quarter of the budget
+    // stays free: a stream of ever larger inputs (a served job mix) then pays a
+    // few reallocations (a device synchronisation, a free and an allocation of
+    // gigabytes each), not one per new largest input
+    if (a.bytes && 2 * a.bytes > want && ctx->held - a.bytes + 2 * a.bytes <= ctx->budget / 4 * 3) want = 2 * a.bytes;
     if (ctx->held - a.bytes + want > ctx->budget) {
         want = bytes;
         if (ctx->held - a.bytes + want > ctx->budget)
