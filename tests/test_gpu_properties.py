@@ -108,12 +108,11 @@ def test_scan_labels_are_deterministic_and_compressed():
 
 
 @pytest.mark.gpu
-def test_wave_per_record_kernels_cover_every_read(tmp_path):
-    # 4.6 M reads: more records than the capped wave-per-record grids hold
-    # (2^20 workgroups x 4 waves), so the packing copy and the writer's record
-    # copy loop over records; the reads past that point must pack and write
-    # like the others
-    print("gpu wave-per-record run: 4.6M reads", flush=True)
+def test_packing_and_writer_cover_every_read(tmp_path):
+    # 4.6 M reads (more than 2^22, where an earlier wave-per-record grid had
+    # to loop): the reads of the last chunk must pack like the same reads
+    # loaded alone, and the writer must copy every record byte once
+    print("gpu packing/writer run: 4.6M reads", flush=True)
     n, tail = 4_600_000, 50_000
     buf = mp.synth(0, n_reads=n, read_len=40, n_genomes=50, genome_len_min=20000, genome_len_max=40000, seed=31)
     v = np.frombuffer(buf.tobytes(), dtype=np.uint8)
