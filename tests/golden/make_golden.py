@@ -37,6 +37,9 @@ SYNTH_INPUTS = {
               "--interleaved"],
     "gc_rep": ["--preset", "0", "--reads", "40000", "--genomes", "10", "--glen-min", "20000", "--glen-max", "60000",
                "--gc-min", "0.3", "--gc-max", "0.7", "--repeats", "3", "--len", "150", "--err", "0.01", "--seed", "21"],
+    # 5 % lower-case bases, 0.5 % other IUPAC letters or a lower-case 'n', 'N's
+    "mixed": ["--preset", "0", "--reads", "20000", "--genomes", "6", "--len-min", "90", "--len-max", "150",
+              "--err", "0.01", "--nrate", "0.002", "--seed", "7", "--mix", "0.05"],
 }
 CASES = [
     ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
@@ -56,6 +59,10 @@ CASES = [
     ("il_min3", "il150", 4, 27, 1, 1, "il", "min"),
     ("gcrep_k31", "gc_rep", 16, 31, 1, 1, "se", ""),
     ("gcrep_k31_S4T2", "gc_rep", 16, 31, 4, 2, "se", ""),
+    ("mixed_k31", "mixed", 4, 31, 1, 1, "se", ""),
+    ("mixed_k27_S2T2", "mixed", 6, 27, 2, 2, "se", ""),
+    ("mixed_k32", "mixed", 4, 32, 1, 1, "se", ""),
+    ("mixed_min3", "mixed", 4, 27, 1, 1, "se", "min"),
 ]
 
 
