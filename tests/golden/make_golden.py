@@ -40,6 +40,9 @@ SYNTH_INPUTS = {
     # 5 % lower-case bases, 0.5 % other IUPAC letters or a lower-case 'n', 'N's
     "mixed": ["--preset", "0", "--reads", "20000", "--genomes", "6", "--len-min", "90", "--len-max", "150",
               "--err", "0.01", "--nrate", "0.002", "--seed", "7", "--mix", "0.05"],
+    # reads of 10-45 bases, 2 % 'N': many reads at or below k, and windows cut by 'N'
+    "short": ["--preset", "0", "--reads", "30000", "--genomes", "4", "--len-min", "10", "--len-max", "45",
+              "--err", "0.01", "--nrate", "0.02", "--seed", "9"],
 }
 CASES = [
     ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
@@ -63,6 +66,10 @@ CASES = [
     ("mixed_k27_S2T2", "mixed", 6, 27, 2, 2, "se", ""),
     ("mixed_k32", "mixed", 4, 32, 1, 1, "se", ""),
     ("mixed_min3", "mixed", 4, 27, 1, 1, "se", "min"),
+    ("var_k13_S2T2", "var", 4, 13, 2, 2, "se", ""),   # k < 20: every sort pass global
+    ("var_k19", "var", 4, 19, 1, 1, "se", ""),
+    ("short_k21", "short", 4, 21, 1, 1, "se", ""),
+    ("short_min3_k21", "short", 4, 21, 1, 1, "se", "min"),
 ]
 
 
