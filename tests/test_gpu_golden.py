@@ -1,8 +1,9 @@
 """The drop-in GPU tools (genFastqIdx, genHistograms_metapart[min],
 metapart[min] over libmetapart_amd) against the real reference's output hashes,
-file by file, on every golden case: plain and 3-minimizer, k = 21..32, one to
+file by file, on every golden case: plain and 3-minimizer, k = 13..32, one to
 four scans, one to three segments, single-end and interleaved input, reads
-with 'N', variable read lengths, the reference's own data/reads.fastq."""
+with 'N', variable read lengths, reads at or below k, lower-case and other
+IUPAC letters, the reference's own data/reads.fastq."""
 import pytest
 
 from _helpers import diff_hashes, golden, run_chain
