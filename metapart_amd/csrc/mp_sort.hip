@@ -106,11 +106,13 @@ __device__ __forceinline__ uint32_t group_of(const uint64_t *first, int n, uint3
     return (uint32_t)lo;
 }
 
+// digit counts of the passes in pass_mask (and of the sample digit when there are samples)
 __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__restrict__ keys,
                                                            const uint8_t *__restrict__ adj,
                                                            const uint32_t *__restrict__ lab,
                                                            const uint64_t *__restrict__ grp_first, int n_grp,
-                                                           uint64_t n, unsigned long long *__restrict__ ghist)
+                                                           uint64_t n, uint32_t pass_mask,
+                                                           unsigned long long *__restrict__ ghist)
 {
     __shared__ uint32_t h[kPasses][kRadix];
     for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) (&h[0][0])[j] = 0;
@@ -118,9 +120,10 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint64_t k = keys[i];
-        atomicAdd(&h[0][adj[i]], 1u);
+        if (pass_mask & 1) atomicAdd(&h[0][adj[i]], 1u);
 #pragma unroll
-        for (int b = 0; b < 8; b++) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
+        for (int b = 0; b < 8; b++)
+            if ((pass_mask >> (1 + b)) & 1) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
         if (grp_first) atomicAdd(&h[9][group_of(grp_first, n_grp, lab[i])], 1u);
     }
     __syncthreads();
@@ -479,7 +482,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)blocks), dim3(kRadix), 0, ctx->stream,
                            (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p,
-                           (const uint32_t *)lab[*cur].p, d_grp_first, n_grp, n, gh);
+                           (const uint32_t *)lab[*cur].p, d_grp_first, n_grp, n, pass_mask, gh);
         MP_HIP(hipGetLastError());
     }
     hipEventRecord(ctx->pev[1], ctx->stream);
