@@ -190,6 +190,13 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
     const int wave = tid / kWave;
     const unsigned lane = __lane_id();
     for (int j = tid; j < kSW * kRadix; j += kSB) (&s_cnt[0][0])[j] = 0;
+    // the sample pass searches the samples' first read ids (at most kRadix) in
+    // LDS: s_gbase is idle until the look-back
+    const uint64_t *grp_first = P.grp_first;
+    if (!GEN && P.shift == -2) {
+        for (int j = tid; j < P.n_grp; j += kSB) s_gbase[j] = P.grp_first[j];
+        grp_first = s_gbase;
+    }
     if (tid == 0) s_tile = atomicAdd(P.tile_ctr, 1u);
     __syncthreads();
     const uint64_t tile = s_tile;
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 l[i] = P.lab_in[idx];
                 const uint8_t a = P.adj_in[idx];
                 ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
-                dr[i] = (P.shift == -2 ? group_of(P.grp_first, P.n_grp, l[i]) : digit_of(k[i], a, P.shift)) << 16;
+                dr[i] = (P.shift == -2 ? group_of(grp_first, P.n_grp, l[i]) : digit_of(k[i], a, P.shift)) << 16;
             } else {
                 k[i] = 0;
                 l[i] = 0;
