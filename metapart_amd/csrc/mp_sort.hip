@@ -115,16 +115,22 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
                                                            unsigned long long *__restrict__ ghist)
 {
     __shared__ uint32_t h[kPasses][kRadix];
+    __shared__ uint64_t s_first[kRadix];  // the samples' first read ids (at most kRadix samples)
     for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) (&h[0][0])[j] = 0;
+    if (grp_first)
+        for (int j = threadIdx.x; j < n_grp; j += blockDim.x) s_first[j] = grp_first[j];
     __syncthreads();
+    const bool keyed = (pass_mask >> 1) & 0xFF;  // only the arrays a counted digit needs are read
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint64_t k = keys[i];
         if (pass_mask & 1) atomicAdd(&h[0][adj[i]], 1u);
+        if (keyed) {
+            const uint64_t k = keys[i];
 #pragma unroll
-        for (int b = 0; b < 8; b++)
-            if ((pass_mask >> (1 + b)) & 1) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
-        if (grp_first) atomicAdd(&h[9][group_of(grp_first, n_grp, lab[i])], 1u);
+            for (int b = 0; b < 8; b++)
+                if ((pass_mask >> (1 + b)) & 1) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
+        }
+        if (grp_first) atomicAdd(&h[9][group_of(s_first, n_grp, lab[i])], 1u);
     }
     __syncthreads();
     for (int j = threadIdx.x; j < kPasses * kRadix; j += blockDim.x) {
