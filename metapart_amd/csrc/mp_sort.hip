@@ -148,27 +148,47 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
 // covering windows [t * tile, (t + 1) * tile) in read order; a window that
 // holds an 'N' or lies outside the scan's buckets is not an item.  The
 // tuples come out in the same order as an enumeration followed by this pass.
+// What a tile of the generating pass needs before it stages its reads, in
+// one 64-byte line so that it costs one round trip: the reads [ra, rb] that
+// overlap its windows, the first window of read ra, the reads' bases
+// [b0, b1) and their FASTQ sequence span [f0, f1).
+struct TileInfo {
+    uint64_t ra, rb, pa, b0, b1, f0, f1, pad;
+};
+
 struct GenPass {
     KParams G;
-    const uint64_t *tile_r0;  // read holding each tile's first window
+    const TileInfo *info;  // per tile
     uint64_t tiles;
-    uint64_t npos;            // windows of all reads
+    uint64_t npos;         // windows of all reads
 };
 constexpr int kGenReads = 256;  // reads of a tile staged in LDS (more: everything read from memory)
 
-// tile_r0[t] = the read holding window t * tile: last r with pos_off[r] <= t * tile
-__global__ void tile_read_kernel(const uint64_t *__restrict__ pos_off, uint64_t N, uint64_t tile, uint64_t tiles,
-                                 uint64_t *__restrict__ tile_r0)
+// read holding window q: last r with pos_off[r] <= q
+__device__ __forceinline__ uint64_t read_of_window(const uint64_t *pos_off, uint64_t N, uint64_t q)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tiles) return;
-    const uint64_t q = t * tile;
     uint64_t lo = 0, hi = N - 1;
     while (lo < hi) {
         const uint64_t mid = (lo + hi + 1) >> 1;
         if (pos_off[mid] <= q) lo = mid; else hi = mid - 1;
     }
-    tile_r0[t] = lo;
+    return lo;
+}
+
+__global__ void tile_info_kernel(KParams G, uint64_t tile, uint64_t tiles, TileInfo *__restrict__ info)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    TileInfo ti;
+    ti.ra = read_of_window(G.pos_off, G.N, t * tile);
+    ti.rb = t + 1 < tiles ? read_of_window(G.pos_off, G.N, (t + 1) * tile) : G.N - 1;
+    ti.pa = G.pos_off[ti.ra];
+    ti.b0 = G.base_off[ti.ra];
+    ti.b1 = G.base_off[ti.rb + 1];
+    ti.f0 = G.seq_off[ti.ra];
+    ti.f1 = G.seq_off[ti.rb] + G.seq_len[ti.rb];
+    ti.pad = 0;
+    info[t] = ti;
 }
 
 // waves per SIMD the tile shape's LDS allows: two 1024-lane tiles, or four 512 x 6 tiles, per CU
@@ -218,11 +238,12 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         // reads [ra, rb] overlap the tile's windows.  When they fit, their table,
         // their 2-bit and 'N' words and their FASTQ sequence bytes are staged in
         // the (idle) staging buffer, so the items read LDS only; else memory.
-        const uint64_t ra = A.tile_r0[tile], rb = tile + 1 < A.tiles ? A.tile_r0[tile + 1] : A.G.N - 1;
+        const TileInfo ti = A.info[tile];
+        const uint64_t ra = ti.ra, rb = ti.rb;
         const uint32_t nr = (uint32_t)(rb - ra + 1);
-        const uint64_t pa = A.G.pos_off[ra];
-        const uint64_t b0 = A.G.base_off[ra], b1 = A.G.base_off[rb + 1];   // bases of the reads
-        const uint64_t f0 = A.G.seq_off[ra], f1 = A.G.seq_off[rb] + A.G.seq_len[rb];  // their FASTQ span
+        const uint64_t pa = ti.pa;
+        const uint64_t b0 = ti.b0, b1 = ti.b1;  // bases of the reads
+        const uint64_t f0 = ti.f0, f1 = ti.f1;  // their FASTQ span
         const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
         const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
         const uint64_t fw0 = f0 >> 2, nf = (f1 + 3) / 4 - fw0;
@@ -558,17 +579,18 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
         ctx->epoch = 0;
     }
-    uint64_t *d_base = nullptr, *d_tile_r0 = nullptr;
+    uint64_t *d_base = nullptr;
+    TileInfo *d_tile_info = nullptr;
     MP_HIP(hipMallocAsync((void **)&d_base, sizeof(base), ctx->stream));
     MP_HIP(hipMemcpyAsync(d_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
     GenPass A{};
     if (gen) {
-        MP_HIP(hipMallocAsync((void **)&d_tile_r0, gtiles * sizeof(uint64_t), ctx->stream));
-        hipLaunchKernelGGL(tile_read_kernel, dim3((unsigned)((gtiles + 255) / 256)), dim3(256), 0, ctx->stream,
-                           gen->P.pos_off, gen->P.N, kTile, gtiles, d_tile_r0);
+        MP_HIP(hipMallocAsync((void **)&d_tile_info, gtiles * sizeof(TileInfo), ctx->stream));
+        hipLaunchKernelGGL(tile_info_kernel, dim3((unsigned)((gtiles + 255) / 256)), dim3(256), 0, ctx->stream,
+                           gen->P, kTile, gtiles, d_tile_info);
         MP_HIP(hipGetLastError());
         A.G = gen->P;
-        A.tile_r0 = d_tile_r0;
+        A.info = d_tile_info;
         A.tiles = gtiles;
         A.npos = gen->npos;
     }
@@ -610,7 +632,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         executed++;
     }
     MP_HIP(hipFreeAsync(d_base, ctx->stream));
-    if (d_tile_r0) MP_HIP(hipFreeAsync(d_tile_r0, ctx->stream));
+    if (d_tile_info) MP_HIP(hipFreeAsync(d_tile_info, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
     float tot = 0, ms = 0;
     ctx->sort_gen_ms = 0;
