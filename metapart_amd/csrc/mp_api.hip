@@ -192,13 +192,15 @@ int mp_count_loaded(mp_ctx *ctx, uint32_t *counts, uint64_t *hist, uint64_t *rea
     MP_TRY(count_chunks_device(ctx, ctx->chunk_ids.data(), C, d_counts));
     uint64_t *d_tot = (uint64_t *)ctx->hist.p, *d_pre = d_tot + MP_NBUCKETS;
     MP_TRY(bucket_totals(ctx, d_counts, C, d_tot));
-    std::vector<uint64_t> pre(MP_NBUCKETS + 1);
     if (counts) MP_HIP(hipMemcpyAsync(counts, d_counts, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     MP_TRY(scan_u64(ctx, d_tot, d_pre, MP_NBUCKETS));
-    MP_HIP(hipMemcpyAsync(pre.data(), d_pre, sizeof(uint64_t) * (MP_NBUCKETS + 1), hipMemcpyDeviceToHost, ctx->stream));
+    // straight into the caller's buffers
+    uint64_t total = 0;
+    if (hist)
+        MP_HIP(hipMemcpyAsync(hist, d_pre, sizeof(uint64_t) * (MP_NBUCKETS + 1), hipMemcpyDeviceToHost, ctx->stream));
+    MP_HIP(hipMemcpyAsync(&total, d_pre + MP_NBUCKETS, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
-    if (hist) memcpy(hist, pre.data(), sizeof(uint64_t) * (MP_NBUCKETS + 1));
-    if (total_kmers) *total_kmers = pre[MP_NBUCKETS];
+    if (total_kmers) *total_kmers = total;
     if (reads_per_chunk)
         for (int c = 0; c < C; c++) reads_per_chunk[c] = ctx->chunk_ids[c + 1] - ctx->chunk_ids[c];
     return 0;
