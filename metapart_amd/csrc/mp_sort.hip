@@ -666,13 +666,10 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
 constexpr int kLWin = MP_LOCAL_WIN, kLMax = MP_LOCAL_WIN, kLCap = kLWin + kLMax, kLThreads = 256;
 constexpr int kLMarg = 256;  // positions loaded past the window for the next group start
 
-struct LocalPass {
-    const uint64_t *keys_in;
-    const uint32_t *lab_in;
-    const uint8_t *adj_in;
-    uint64_t *keys_out;
-    uint32_t *lab_out;
-    uint8_t *adj_out;
+struct LocalPass {  // the tuple arrays, sorted in place
+    uint64_t *keys;
+    uint32_t *lab;
+    uint8_t *adj;
     uint64_t n;
     int shift;                  // high bits: key >> shift
     uint64_t *ovf;              // starts of groups longer than kLMax
@@ -697,9 +694,9 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     // the window and a margin after it, in one coalesced load; group starts
     // from LDS (the first position of the array is a start)
     const uint32_t nld = (uint32_t)(min(w1 + kLMarg, lim) - w0);
-    for (uint32_t j = threadIdx.x; j < nld; j += kLThreads) s_key[j] = L.keys_in[w0 + j];
+    for (uint32_t j = threadIdx.x; j < nld; j += kLThreads) s_key[j] = L.keys[w0 + j];
     if (threadIdx.x == 0) {
-        s_prev = w0 ? L.keys_in[w0 - 1] : 0;
+        s_prev = w0 ? L.keys[w0 - 1] : 0;
         s_r1 = lim;
     }
     __syncthreads();
@@ -733,7 +730,7 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     if (r1 >= nld && w0 + nld < lim) {
         for (uint64_t c0 = w0 + nld; c0 < lim; c0 += kLThreads) {
             const uint64_t i = c0 + threadIdx.x;
-            const bool st = i < lim && group_start(L.keys_in, i, L.shift);
+            const bool st = i < lim && group_start(L.keys, i, L.shift);
             const uint64_t m = __ballot(st);
             if (lane == 0 && m) atomicMin(&s_r1, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
             if (__syncthreads_or(st)) break;
@@ -750,7 +747,7 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
             r1 = last;
             if (threadIdx.x == 0) L.ovf[atomicAdd(L.novf, 1ull)] = w0 + last;
         } else {  // the rest of the window's last group, and no start inside it
-            for (uint32_t j = nld + threadIdx.x; j < r1; j += kLThreads) s_key[j] = L.keys_in[w0 + j];
+            for (uint32_t j = nld + threadIdx.x; j < r1; j += kLThreads) s_key[j] = L.keys[w0 + j];
             for (uint32_t w = nwl + threadIdx.x; w * kWave < r1; w += kLThreads) s_st[w] = 0;
             __syncthreads();
         }
@@ -758,23 +755,28 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     // groups [r0, r1): each tuple's rank inside its group by (low bits, position)
     const uint32_t m = r1 - r0;
     constexpr int kQ = kLCap / kLThreads;
-    uint64_t kr[kQ], c[kQ];  // keys, and (low bits << 12) | index: the order within a group, ties by position
+    uint64_t kr[kQ];  // keys; the order keys (low bits << 12) | index go to LDS
     const uint64_t lowmask = (1ull << L.shift) - 1;
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
-        if (j < m) {
-            kr[q] = s_key[r0 + j];
-            c[q] = ((kr[q] & lowmask) << 12) | j;
-        }
+        if (j < m) kr[q] = s_key[r0 + j];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
-        if (j < m) s_key[r0 + j] = c[q];
+        if (j < m) s_key[r0 + j] = ((kr[q] & lowmask) << 12) | j;
     }
     __syncthreads();
+    // The pass runs in place: a tuple already at its rank (every tuple of a
+    // group that holds one k-mer) is not written, and a tuple that moves reads
+    // its label and adj before the barrier and is written after it.  The
+    // workgroup owns its groups' positions; its neighbours read keys there
+    // only for their high bits, which a permutation inside a group keeps.
+    uint32_t rk[kQ], lb[kQ];  // rank | adj << 24 (~0: stays), label
+#pragma unroll
+    for (int q = 0; q < kQ; q++) rk[q] = ~0u;
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         const uint32_t j = q * kLThreads + threadIdx.x;
@@ -791,13 +793,22 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         uint64_t be = s_st[w] & ~upto;
         while (!be && ++w * kWave < r1) be = s_st[w];
         const uint32_t ge = be ? min(r1, w * kWave + (uint32_t)__builtin_ctzll(be)) : r1;
-        const uint64_t cj = c[q];
+        const uint64_t cj = s_key[p];
         uint32_t rank = gs;
         for (uint32_t t = gs; t < ge; t++) rank += s_key[t] < cj;
-        const uint64_t src = w0 + p, dst = w0 + rank;
-        L.keys_out[dst] = kr[q];
-        L.lab_out[dst] = L.lab_in[src];
-        L.adj_out[dst] = L.adj_in[src];
+        if (rank != p) {
+            rk[q] = rank | ((uint32_t)L.adj[w0 + p] << 24);  // rank < kLCap
+            lb[q] = L.lab[w0 + p];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+        if (rk[q] == ~0u) continue;
+        const uint64_t dst = w0 + (rk[q] & 0xFFFFFFu);
+        L.keys[dst] = kr[q];
+        L.lab[dst] = lb[q];
+        L.adj[dst] = (uint8_t)(rk[q] >> 24);
     }
 }
 
@@ -850,14 +861,11 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     MP_TRY(ensure(ctx, ctx->scratch, (wins + 2) * sizeof(uint64_t) + 64));
     unsigned long long *novf = (unsigned long long *)ctx->scratch.p;
     MP_TRY(fill_u32(ctx, novf, 0, 2));
-    const int src = ctx->cur, dst = 1 - ctx->cur;
+    const int src = ctx->cur;  // the local pass sorts this buffer in place
     LocalPass L;
-    L.keys_in = (const uint64_t *)ctx->keys[src].p;
-    L.lab_in = (const uint32_t *)ctx->lab[src].p;
-    L.adj_in = (const uint8_t *)ctx->adj[src].p;
-    L.keys_out = (uint64_t *)ctx->keys[dst].p;
-    L.lab_out = (uint32_t *)ctx->lab[dst].p;
-    L.adj_out = (uint8_t *)ctx->adj[dst].p;
+    L.keys = (uint64_t *)ctx->keys[src].p;
+    L.lab = (uint32_t *)ctx->lab[src].p;
+    L.adj = (uint8_t *)ctx->adj[src].p;
     L.n = n;
     L.shift = S;
     L.ovf = (uint64_t *)ctx->scratch.p + 1;
@@ -881,7 +889,7 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         uint64_t *d_ends = nullptr;
         MP_HIP(hipMallocAsync((void **)&d_ends, nl * sizeof(uint64_t), ctx->stream));
         hipLaunchKernelGGL(long_end_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream,
-                           L.keys_in, n, S, (const uint64_t *)L.ovf, nl, d_ends);
+                           (const uint64_t *)L.keys, n, S, (const uint64_t *)L.ovf, nl, d_ends);
         MP_HIP(hipGetLastError());
         MP_HIP(hipMemcpyAsync(se.data(), L.ovf, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         MP_HIP(hipMemcpyAsync(se.data() + nl, d_ends, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -904,10 +912,10 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
             }
             int c = src, np = 0;
             MP_TRY(radix_sort_generic(ctx, kv, lv, av, &c, e - a, lmask, &np, nullptr, 0, nullptr));
-            if (c != dst) {  // an even number of passes ran: the range is still in the source buffer
-                MP_HIP(hipMemcpyAsync(kv[dst].p, kv[src].p, (e - a) * 8, hipMemcpyDeviceToDevice, ctx->stream));
-                MP_HIP(hipMemcpyAsync(lv[dst].p, lv[src].p, (e - a) * 4, hipMemcpyDeviceToDevice, ctx->stream));
-                MP_HIP(hipMemcpyAsync(av[dst].p, av[src].p, e - a, hipMemcpyDeviceToDevice, ctx->stream));
+            if (c != src) {  // an odd number of passes ran: the range is in the other buffer
+                MP_HIP(hipMemcpyAsync(kv[src].p, kv[c].p, (e - a) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                MP_HIP(hipMemcpyAsync(lv[src].p, lv[c].p, (e - a) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                MP_HIP(hipMemcpyAsync(av[src].p, av[c].p, e - a, hipMemcpyDeviceToDevice, ctx->stream));
             }
         }
         MP_HIP(hipStreamSynchronize(ctx->stream));
@@ -915,7 +923,6 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         ctx->sort_hist_ms = keep_hist;
         ctx->sort_gen_ms = keep_gen;
     }
-    ctx->cur = dst;
     *passes_out += 1;
     return 0;
 }
