@@ -174,10 +174,15 @@ def main():
     pass_ms = float(sc["ms_sort_pass_avg"])
     per_pass_bytes = 2 * TUPLE_BYTES * n  # read and write 13 B per tuple
     achieved = per_pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    # the local pass (when it ran) reads every key and moves only the tuples not
+    # yet at their rank: their label and adj are read and the tuple written
+    local = float(sc["ms_sort_local"]) > 0
+    local_bytes = n * 8 + int(sc["sort_local_moved"]) * (5 + TUPLE_BYTES) if local else 0
+    full = passes - (1 if local else 0)  # radix passes over the whole array
     if float(sc["ms_sort_gen"]) > 0:  # the first pass generates the tuples: it writes them and reads the reads
-        sort_bytes = n * TUPLE_BYTES + n * 2 * (passes - 1) * TUPLE_BYTES
+        sort_bytes = n * TUPLE_BYTES + n * 2 * (full - 1) * TUPLE_BYTES + local_bytes
     else:  # enumerated tuples: one histogram read (k-mers), then every pass reads and writes them
-        sort_bytes = n * (8 + 2 * passes * TUPLE_BYTES)
+        sort_bytes = n * (8 + 2 * full * TUPLE_BYTES) + local_bytes
     sort_ms = float(sc["ms_sort"])
     traffic = measured_traffic(args.reads)
     out = {

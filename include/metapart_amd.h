@@ -88,6 +88,7 @@ typedef struct {
     uint64_t sort_long_groups;/* groups too long for the local pass, sorted by their own global passes */
     float ms_sort_gen;        /* the first radix pass, generating the tuples from the reads (0: enumerated) */
     int parts;                /* 0: one pass; else the scan did not fit the budget and went twice over this many bucket parts */
+    uint64_t sort_local_moved;/* tuples the in-place local pass moved (the others were already at their rank) */
 } mp_scan_stats;
 
 typedef struct {

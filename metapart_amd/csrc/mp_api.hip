@@ -390,6 +390,7 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     st.ms_sort_hist = ctx->sort_hist_ms;
     st.ms_sort_local = ctx->local_ms;
     st.sort_long_groups = ctx->local_long;
+    st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen ? ctx->sort_gen_ms : 0.f;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
@@ -590,6 +591,7 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     st.ms_sort_hist = ctx->sort_hist_ms;
     st.ms_sort_local = ctx->local_ms;
     st.sort_long_groups = ctx->local_long;
+    st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen_ms;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);

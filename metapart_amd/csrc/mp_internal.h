@@ -101,6 +101,7 @@ struct mp_ctx {
     float sort_gen_ms = 0;     // the generating radix pass of the last sort (0: none)
     float local_ms = 0;        // local rank pass of the last sort
     uint64_t local_long = 0;   // groups it handed to the long-group kernel
+    uint64_t local_moved = 0;  // tuples it moved (the rest stayed in place)
     std::string err;
     uint64_t budget = 0, held = 0;
     uint16_t epoch = 0;  // look-back status epoch of the radix sort

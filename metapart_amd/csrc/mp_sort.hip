@@ -674,6 +674,7 @@ struct LocalPass {  // the tuple arrays, sorted in place
     int shift;                  // high bits: key >> shift
     uint64_t *ovf;              // starts of groups longer than kLMax
     unsigned long long *novf;
+    uint32_t *moved;            // per wave: tuples written elsewhere than they were (summed afterwards)
 };
 
 __device__ __forceinline__ bool group_start(const uint64_t *keys, uint64_t i, int shift)
@@ -802,6 +803,7 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         }
     }
     __syncthreads();
+    uint32_t nmoved = 0;
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         if (rk[q] == ~0u) continue;
@@ -809,6 +811,29 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         L.keys[dst] = kr[q];
         L.lab[dst] = lb[q];
         L.adj[dst] = (uint8_t)(rk[q] >> 24);
+        nmoved++;
+    }
+    // one plain store per wave (no barrier, no atomic: millions of adds on a
+    // few addresses serialise)
+    for (int d = kWave / 2; d > 0; d >>= 1) nmoved += __shfl_down(nmoved, d, kWave);
+    if (lane == 0) L.moved[(uint64_t)blockIdx.x * (kLThreads / kWave) + wave] = nmoved;
+}
+
+// sum of n counts into *out (grid-stride, one atomic per workgroup)
+__global__ __launch_bounds__(256) void sum_u32_kernel(const uint32_t *__restrict__ v, uint64_t n,
+                                                      unsigned long long *__restrict__ out)
+{
+    uint64_t t = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) t += v[i];
+    for (int d = kWave / 2; d > 0; d >>= 1) t += __shfl_down(t, d, kWave);
+    __shared__ uint64_t s[256 / kWave];
+    if (__lane_id() == 0) s[threadIdx.x / kWave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0;
+        for (int w = 0; w < 256 / kWave; w++) a += s[w];
+        if (a) atomicAdd(out, (unsigned long long)a);
     }
 }
 
@@ -858,9 +883,14 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
                               nullptr, gen));
     if (S == 0 || n <= 1) return 0;
     const uint64_t wins = (n + kLWin - 1) / kLWin;
-    MP_TRY(ensure(ctx, ctx->scratch, (wins + 2) * sizeof(uint64_t) + 64));
+    // scratch: long-group count, moved-tuple total, long-group starts (one per window at most),
+    // then the per-wave moved counts
+    const uint64_t nwave = wins * (kLThreads / kWave);
+    MP_TRY(ensure(ctx, ctx->scratch, (wins + 3) * sizeof(uint64_t) + nwave * sizeof(uint32_t) + 64));
     unsigned long long *novf = (unsigned long long *)ctx->scratch.p;
-    MP_TRY(fill_u32(ctx, novf, 0, 2));
+    uint32_t *d_moved = (uint32_t *)((uint64_t *)ctx->scratch.p + wins + 3);
+    MP_TRY(fill_u32(ctx, novf, 0, 4));
+    MP_TRY(fill_u32(ctx, d_moved, 0, nwave));
     const int src = ctx->cur;  // the local pass sorts this buffer in place
     LocalPass L;
     L.keys = (uint64_t *)ctx->keys[src].p;
@@ -868,8 +898,9 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     L.adj = (uint8_t *)ctx->adj[src].p;
     L.n = n;
     L.shift = S;
-    L.ovf = (uint64_t *)ctx->scratch.p + 1;
+    L.ovf = (uint64_t *)ctx->scratch.p + 2;
     L.novf = novf;
+    L.moved = d_moved;
     if (!grid_fits(wins, kLThreads))
         return ctx->fail(MP_EINVAL, "local pass of %llu windows exceeds one dispatch: use more scans",
                          (unsigned long long)wins);
@@ -877,9 +908,14 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     hipLaunchKernelGGL(local_rank_kernel, dim3((unsigned)wins), dim3(kLThreads), 0, ctx->stream, L);
     MP_HIP(hipGetLastError());
     hipEventRecord(ctx->pev[23], ctx->stream);
-    uint64_t nl = 0;
-    MP_HIP(hipMemcpyAsync(&nl, novf, sizeof(nl), hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(sum_u32_kernel, dim3(stride_grid(nwave)), dim3(256), 0, ctx->stream, (const uint32_t *)d_moved,
+                       nwave, novf + 1);
+    MP_HIP(hipGetLastError());
+    uint64_t cnt[2] = {0, 0};
+    MP_HIP(hipMemcpyAsync(cnt, novf, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t nl = cnt[0];
+    ctx->local_moved = cnt[1];
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->pev[22], ctx->pev[23]);
     ctx->local_ms = ms;
