@@ -87,9 +87,17 @@ __global__ __launch_bounds__(kParseBlock) void nl_write_kernel(const uint8_t *__
         for (int i = 0; i < 64; i++)
             if (i < n) w[i >> 2] |= (uint32_t)fq[base + i] << (8 * (i & 3));
     }
-    int c = 0;
+    // bit i of nl: byte i is a newline (exact zero-byte test on each word xor '\n')
+    uint64_t nl = 0;
 #pragma unroll
-    for (int i = 0; i < 64; i++) c += (i < n) && ((w[i >> 2] >> (8 * (i & 3))) & 0xFF) == '\n';
+    for (int i = 0; i < 16; i++) {
+        const uint32_t x = w[i] ^ 0x0A0A0A0Au;
+        const uint32_t y = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);  // 0x80 where the byte is 0
+        const uint32_t nib = ((y >> 7) & 1u) | ((y >> 14) & 2u) | ((y >> 21) & 4u) | ((y >> 28) & 8u);
+        nl |= (uint64_t)nib << (4 * i);
+    }
+    if (n < 64) nl &= (1ull << n) - 1;
+    const int c = __popcll(nl);
     // exclusive prefix of c over the block
     const unsigned lane = __lane_id();
     int inc = c;
@@ -103,10 +111,8 @@ __global__ __launch_bounds__(kParseBlock) void nl_write_kernel(const uint8_t *__
     int woff = 0;
     for (int w = 0; w < (int)(threadIdx.x / kWave); w++) woff += s[w];
     uint64_t j = blk_off[blockIdx.x] + (uint64_t)(woff + inc - c);
-#pragma unroll
-    for (int i = 0; i < 64; i++) {
-        if (i >= n || ((w[i >> 2] >> (8 * (i & 3))) & 0xFF) != '\n') continue;
-        const uint64_t pos = base + (uint64_t)i;
+    for (uint64_t m = nl; m; m &= m - 1) {
+        const uint64_t pos = base + (uint64_t)__builtin_ctzll(m);
         const uint64_t r = j >> 2;
         if (r < n_rec) {
             switch (j & 3) {
