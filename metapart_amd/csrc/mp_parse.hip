@@ -61,12 +61,15 @@ __global__ __launch_bounds__(kParseBlock) void nl_count_kernel(const uint8_t *__
 
 // Writes, for newline j at byte p: j%4==0 -> seq_off[j/4] = p+1;
 // j%4==1 -> seq_end[j/4] = p; j%4==2 -> plus_end[j/4] = p; j%4==3 -> rec_off[j/4+1] = p+1.
+// It also checks the byte after the sequence line ('+') and after a record
+// ('@' of the next record): first_bad takes the lowest record that fails.
 __global__ __launch_bounds__(kParseBlock) void nl_write_kernel(const uint8_t *__restrict__ fq, uint64_t nbytes,
                                                                const uint64_t *__restrict__ blk_off, uint64_t n_rec,
                                                                uint64_t *__restrict__ seq_off,
                                                                uint64_t *__restrict__ seq_end,
                                                                uint64_t *__restrict__ plus_end,
-                                                               uint64_t *__restrict__ rec_off)
+                                                               uint64_t *__restrict__ rec_off,
+                                                               unsigned long long *__restrict__ first_bad)
 {
     const uint64_t base = (uint64_t)blockIdx.x * kParseBlockBytes + (uint64_t)threadIdx.x * kParseBytesPerThread;
     uint32_t w[16];
@@ -112,7 +115,8 @@ __global__ __launch_bounds__(kParseBlock) void nl_write_kernel(const uint8_t *__
     for (int w = 0; w < (int)(threadIdx.x / kWave); w++) woff += s[w];
     uint64_t j = blk_off[blockIdx.x] + (uint64_t)(woff + inc - c);
     for (uint64_t m = nl; m; m &= m - 1) {
-        const uint64_t pos = base + (uint64_t)__builtin_ctzll(m);
+        const int i = __builtin_ctzll(m);
+        const uint64_t pos = base + (uint64_t)i;
         const uint64_t r = j >> 2;
         if (r < n_rec) {
             switch (j & 3) {
@@ -120,6 +124,20 @@ __global__ __launch_bounds__(kParseBlock) void nl_write_kernel(const uint8_t *__
             case 1: seq_end[r] = pos; break;
             case 2: plus_end[r] = pos; break;
             default: rec_off[r + 1] = pos + 1; break;
+            }
+            if ((j & 3) == 1 || ((j & 3) == 3 && r + 1 < n_rec)) {
+                uint32_t nb;  // the byte after the newline
+                if (i + 1 < n) {
+                    const int k = i + 1;
+                    uint32_t wk = 0;
+#pragma unroll
+                    for (int t = 0; t < 16; t++) wk = t == (k >> 2) ? w[t] : wk;
+                    nb = (wk >> (8 * (k & 3))) & 0xFF;
+                } else {
+                    nb = pos + 1 < nbytes ? fq[pos + 1] : 0u;
+                }
+                const uint32_t want = (j & 3) == 1 ? '+' : '@';
+                if (nb != want) atomicMin(first_bad, (unsigned long long)((j & 3) == 1 ? r : r + 1));
             }
         }
         j++;
@@ -140,7 +158,8 @@ __global__ void rec_finalize_kernel(const uint8_t *__restrict__ fq, uint64_t n_r
     const uint64_t so = seq_off[r], se = seq_end[r], pe = plus_end[r], re = rec_off[r + 1], rs = rec_off[r];
     const uint64_t L = se - so;
     const uint64_t qual = re - 1 - (pe + 1);
-    bool ok = fq[rs] == '@' && fq[se + 1] == '+' && qual == L && L < 65536;
+    // the '+' line and the '@' of every later record were checked by nl_write_kernel
+    bool ok = (r != 0 || fq[rs] == '@') && qual == L && L < 65536;
     if (ok && L > (uint64_t)K && L - K >= 300) ok = false;  // enumerateKmers.c:155
     if (!ok) atomicMin(first_bad, (unsigned long long)r);
     seq_len[r] = (uint32_t)L;
@@ -215,13 +234,13 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     uint32_t *npos = (uint32_t *)ctx->lab[0].p;
     uint64_t *rec_off = (uint64_t *)ctx->rec_off.p;
     MP_TRY(fill_u32(ctx, rec_off, 0, 2));
-    hipLaunchKernelGGL(nl_write_kernel, dim3((unsigned)nblk), dim3(kParseBlock), 0, ctx->stream, fq, nbytes,
-                       (const uint64_t *)ctx->wg_off.p, N, (uint64_t *)ctx->seq_off.p, seq_end, plus_end, rec_off);
-    MP_HIP(hipGetLastError());
-
     MP_TRY(ensure(ctx, ctx->hist, 4096));
     unsigned long long *first_bad = (unsigned long long *)ctx->hist.p;
     MP_TRY(fill_u32(ctx, first_bad, 0xFFFFFFFFu, 2));
+    hipLaunchKernelGGL(nl_write_kernel, dim3((unsigned)nblk), dim3(kParseBlock), 0, ctx->stream, fq, nbytes,
+                       (const uint64_t *)ctx->wg_off.p, N, (uint64_t *)ctx->seq_off.p, seq_end, plus_end, rec_off,
+                       first_bad);
+    MP_HIP(hipGetLastError());
     hipLaunchKernelGGL(rec_finalize_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, fq, N, K,
                        (const uint64_t *)ctx->seq_off.p, seq_end, plus_end, (const uint64_t *)rec_off,
                        (uint32_t *)ctx->seq_len.p, npos, first_bad);
