@@ -85,6 +85,7 @@ __device__ __forceinline__ bool kmer_from(int K, uint64_t b, uint32_t p, NWord n
 struct RollKmer {
     uint64_t F, R;
     uint32_t nrun;
+    uint64_t pi = ~0ull, ni = ~0ull, pw = 0, nw = 0;  // the 2-bit and 'N' words last read, and their indices
     template <typename NWord, typename PWord>
     __device__ __forceinline__ void init(int K, uint64_t b, NWord nword, PWord pword)
     {
@@ -104,8 +105,16 @@ struct RollKmer {
     template <typename NWord, typename PWord>
     __device__ __forceinline__ void roll(int K, uint64_t bn, NWord nword, PWord pword)
     {
-        const uint64_t c = (pword(bn >> 5) >> (62 - 2 * (bn & 31))) & 3;
-        const uint64_t nb = (nword(bn >> 6) >> (63 - (bn & 63))) & 1;
+        if ((bn >> 5) != pi) {  // a word covers 32 (64) bases: most steps reuse it
+            pi = bn >> 5;
+            pw = pword(pi);
+        }
+        if ((bn >> 6) != ni) {
+            ni = bn >> 6;
+            nw = nword(ni);
+        }
+        const uint64_t c = (pw >> (62 - 2 * (bn & 31))) & 3;
+        const uint64_t nb = (nw >> (63 - (bn & 63))) & 1;
         F = (F << 2) | c;
         if (K < 32) F &= (1ull << (2 * K)) - 1;
         R = (R >> 2) | ((3 - c) << (2 * K - 2));
