@@ -159,7 +159,9 @@ __device__ int eval_run_mem(const GParams &G, uint64_t i, uint64_t j)
 // wave owns 64 consecutive tuples, a "word").  Three steps:
 //   1. per word, wave ballots give the run starts (segment starts included),
 //      the tuples with a non-zero k-mer and three bit planes each of the in
-//      and out neighbour classes; the run starts of the tile go to a list;
+//      and out neighbour classes; a run of one tuple (the next tuple of the
+//      word starts a run too) has no verdict, which its lane records at once,
+//      and the tile's other run starts go to a list;
 //   2. one lane per listed run finds the run's end in the start bitmap, its
 //      class tallies by popcounts over the run's words, and its verdict; a run
 //      longer than the tuples looked at is evaluated from memory;
@@ -211,6 +213,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     static_assert(kFIter == 9, "adj bytes kept in two registers");
     uint64_t ap0 = 0;  // this lane's adj bytes, for step 3 (the ninth in ap1)
     uint32_t ap1 = 0;
+    uint32_t ev = 0, z = 0;  // runs evaluated (the last run of a segment is not), runs zeroed
     __syncthreads();
 #pragma unroll 1
     for (int q = 0; q < kFIter; q++) {
@@ -234,10 +237,18 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         for (int b = 0; b < 6; b++) pl[b] = __ballot(valid && ((a >> b) & 1));
         const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
         const uint64_t own = jw + 64 <= cnt ? ~0ull : (jw < cnt ? (1ull << (cnt - jw)) - 1 : 0);
+        // runs of one tuple: the tuple after it (same word) starts a run.  Its
+        // tallies hold one class a side, so no verdict; it is evaluated unless
+        // it ends its segment (it cannot end the tuples: the next one is valid)
+        const uint64_t msingle = mst & (mst >> 1) & own, mlist = mst & own & ~msingle;
+        if ((msingle >> lane) & 1) {
+            s_vd[j] = 0;
+            ev += t0 + j + 1 != s1;
+        }
         uint32_t base = 0;
-        if (lane == 0 && (mst & own)) base = atomicAdd(&s_n, (uint32_t)__popcll(mst & own));
+        if (lane == 0 && mlist) base = atomicAdd(&s_n, (uint32_t)__popcll(mlist));
         base = __shfl(base, 0, kWave);
-        if (st && j < cnt) s_list[base + (uint32_t)__popcll(mst & own & lanemask_lt())] = (uint16_t)j;
+        if ((mlist >> lane) & 1) s_list[base + (uint32_t)__popcll(mlist & lanemask_lt())] = (uint16_t)j;
         if (lane == 0 && jw < cnt_s) {
             const uint32_t wd = jw / kWave;
             s_st[wd] = mst;
@@ -249,7 +260,6 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     }
     __syncthreads();
     const uint32_t nw = (cnt_s + kWave - 1) / kWave, nruns = s_n;
-    uint32_t ev = 0, z = 0;
     for (uint32_t r = threadIdx.x; r < nruns; r += kFLanes) {
         const uint32_t j = s_list[r];
         const uint64_t i = t0 + j;
