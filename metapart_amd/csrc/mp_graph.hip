@@ -214,17 +214,32 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     uint64_t ap0 = 0;  // this lane's adj bytes, for step 3 (the ninth in ap1)
     uint32_t ap1 = 0;
     uint32_t ev = 0, z = 0;  // runs evaluated (the last run of a segment is not), runs zeroed
+    // each iteration's key, previous key and adj byte are loaded one iteration ahead
+    uint64_t nk = 0, nkp = 0;
+    uint32_t na = 0;
+    auto load = [&](int q) {
+        const uint32_t j = q * kFLanes + threadIdx.x;
+        if (j < cnt_s) {
+            const uint64_t i = t0 + j;
+            nk = G.keys[i];
+            nkp = i ? G.keys[i - 1] : 0;
+            na = G.adj[i];
+        }
+    };
+    load(0);
     __syncthreads();
 #pragma unroll 1
     for (int q = 0; q < kFIter; q++) {
         const uint32_t j = q * kFLanes + threadIdx.x;
         const bool valid = j < cnt_s;
+        const uint64_t k = nk, kp = nkp;
+        const uint32_t an = na;
+        if (q + 1 < kFIter) load(q + 1);
         bool st = false, nz = false;
         uint32_t a = 0;
         if (valid) {
             const uint64_t i = t0 + j;
-            const uint64_t k = G.keys[i], kp = i ? G.keys[i - 1] : 0;
-            a = G.adj[i];
+            a = an;
             seg(i);
             st = i == s0 || kp != k;
             nz = k != 0;
@@ -234,7 +249,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         const uint64_t mst = __ballot(st), mnz = __ballot(nz);
         uint64_t pl[6];
 #pragma unroll
-        for (int b = 0; b < 6; b++) pl[b] = __ballot(valid && ((a >> b) & 1));
+        for (int b = 0; b < 6; b++) pl[b] = __builtin_amdgcn_ballot_w64((a >> b) & 1);  // a = 0 past the tuples
         const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
         const uint64_t own = jw + 64 <= cnt ? ~0ull : (jw < cnt ? (1ull << (cnt - jw)) - 1 : 0);
         // runs of one tuple: the tuple after it (same word) starts a run.  Its
