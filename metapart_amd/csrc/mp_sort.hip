@@ -795,8 +795,10 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         while (!be && ++w * kWave < r1) be = s_st[w];
         const uint32_t ge = be ? min(r1, w * kWave + (uint32_t)__builtin_ctzll(be)) : r1;
         const uint64_t cj = s_key[p];
+        // rank = gs + the group's order keys below cj (the LDS address is the
+        // loop variable: one add and one test a step besides the compare)
         uint32_t rank = gs;
-        for (uint32_t t = gs; t < ge; t++) rank += s_key[t] < cj;
+        for (const uint64_t *sp = s_key + gs, *se = s_key + ge; sp < se; sp++) rank += *sp < cj;
         if (rank != p) {
             rk[q] = rank | ((uint32_t)L.adj[w0 + p] << 24);  // rank < kLCap
             lb[q] = L.lab[w0 + p];
