@@ -13,7 +13,9 @@
  *  - host buffers are owned by the caller; device memory is owned by the ctx.
  *  - a ctx is used by one host thread at a time; several ctxs may share a GPU.
  *  - calls are stream-ordered on the ctx's stream and synchronous at return.
- *  - read ids and labels are 32-bit: at most 4,294,967,294 reads per run.
+ *  - read ids and labels are 32-bit: at most 4,294,967,294 reads per run;
+ *    the writer (mp_write_partitions, mp_batch_write) takes at most
+ *    2,147,483,647 reads and returns MP_EINVAL beyond that.
  */
 #ifndef METAPART_AMD_H
 #define METAPART_AMD_H
@@ -251,7 +253,8 @@ int mp_server_submit(mp_server *srv, const char *fq, uint64_t nbytes, int want_p
 
 /* Blocks until the job is done; parent (cap entries, may be NULL) receives
  * its labels over its own read ids if it was submitted with want_parent.
- * A job is waited for once. */
+ * A job is waited for once: a second wait on the same job, from any thread,
+ * returns MP_ESTATE while the first is pending and MP_EINVAL after it. */
 int mp_server_wait(mp_server *srv, uint64_t job, mp_job_result *res, uint32_t *parent, uint64_t cap);
 
 double mp_server_clock(const mp_server *srv);

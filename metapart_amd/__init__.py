@@ -492,38 +492,49 @@ def synth_cfg(preset: int = 0, **over) -> SynthCfg:
 
 def mix_params(n_samples: int = 64, seed: int = 4, reads_min: int = 10_000, reads_max: int = 5_000_000,
                genomes_min: int = 20, genomes_max: int = 300, genome_len_min: int = 100_000,
-               genome_len_max: int = 1_000_000, len_min: int = 100, len_max: int = 150, err_min: float = 0.005,
-               err_max: float = 0.02):
-    """Per-sample generator settings of the 64-sample mix: every sample has its
-    own seeded genome pool (genomes_min..genomes_max genomes), a log-uniform
-    read count (even), one read length in [len_min, len_max] and a uniform
-    error rate in [err_min, err_max]."""
+               genome_len_max: int = 2_000_000, len_min: int = 100, len_max: int = 150, err_min: float = 0.005,
+               err_max: float = 0.02, seed0: int = 100):
+    """Per-sample generator settings of the 64-sample mix: sample i has its own
+    genome pool seeded seed0 + i (genomes_min..genomes_max genomes of
+    genome_len_min..genome_len_max bases, GC fraction U(0.3, 0.7) per genome),
+    a log-uniform read count (even), one read length in [len_min, len_max] and
+    a uniform error rate in [err_min, err_max] (these draws from `seed`)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n_samples):
         n = int(np.exp(rng.uniform(np.log(reads_min), np.log(reads_max))))
         n += n & 1
-        out.append(dict(seed=1000003 * (seed + 1) + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
-                        genome_len_min=genome_len_min, genome_len_max=genome_len_max, abund_sigma=1.0,
-                        n_reads=n, read_len=int(rng.integers(len_min, len_max + 1)),
+        out.append(dict(seed=seed0 + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
+                        genome_len_min=genome_len_min, genome_len_max=genome_len_max, gc_min=0.3, gc_max=0.7,
+                        abund_sigma=1.0, n_reads=n, read_len=int(rng.integers(len_min, len_max + 1)),
                         err_rate=float(rng.uniform(err_min, err_max))))
     return out
 
 
-def job_params(n_jobs: int = 64, seed: int = 7, reads_min: int = 50_000, reads_max: int = 2_000_000,
-               genomes_min: int = 20, genomes_max: int = 100, genome_len_min: int = 100_000,
-               genome_len_max: int = 1_000_000):
-    """Partition jobs of the served workload: log-uniform read counts (even),
-    150 bp reads, 1 % errors, every job with a fresh seeded genome pool."""
+def job_params(n_jobs: int = 200, seed: int = 7, reads_min: int = 50_000, reads_max: int = 2_000_000,
+               genomes_min: int = 20, genomes_max: int = 100, genome_len_min: int = 500_000,
+               genome_len_max: int = 3_000_000, seed0: int = 5000):
+    """Partition jobs of the served workload: job j has a fresh genome pool
+    seeded seed0 + j (genomes_min..genomes_max genomes of genome_len_min..
+    genome_len_max bases), a log-uniform read count (even), 150 bp reads and
+    1 % errors (the counts drawn from `seed`)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n_jobs):
         n = int(np.exp(rng.uniform(np.log(reads_min), np.log(reads_max))))
         n += n & 1
-        out.append(dict(seed=7919 * (seed + 1) + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
+        out.append(dict(seed=seed0 + i, n_genomes=int(rng.integers(genomes_min, genomes_max + 1)),
                         genome_len_min=genome_len_min, genome_len_max=genome_len_max, abund_sigma=1.0,
                         n_reads=n, read_len=150, err_rate=0.01))
     return out
+
+
+def synth_many(params, threads: int = 4):
+    """synth(0, **p) for every p, a few at a time on host threads (each
+    generator call runs its own OpenMP loops and releases the GIL)."""
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda p: synth(0, **p), params))
 
 
 def synth(preset: int = 0, **over) -> SynthBuffer:

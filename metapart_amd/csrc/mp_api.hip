@@ -228,6 +228,12 @@ int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const u
 {
     if (!ctx || !d_fq) return MP_EINVAL;
     MP_HIP(hipSetDevice(ctx->device));
+    // a FASTQ buffer an earlier host load uploaded is no longer needed: it
+    // would count against the budget of every later scan
+    if (ctx->fq_owned) {
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        release(ctx, ctx->fq);
+    }
     ctx->d_fq = (const uint8_t *)d_fq;
     ctx->fq_owned = false;
     ctx->fq_bytes = nbytes;
@@ -519,6 +525,10 @@ int mp_batch_load_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, int n, 
     if (!ctx || !d_fq || n <= 0 || !sample_ofs) return MP_EINVAL;
     if (n > MP_MAX_SAMPLES) return ctx->fail(MP_EINVAL, "at most %d samples per batch", MP_MAX_SAMPLES);
     MP_HIP(hipSetDevice(ctx->device));
+    if (ctx->fq_owned) {
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        release(ctx, ctx->fq);
+    }
     ctx->d_fq = (const uint8_t *)d_fq;
     ctx->fq_owned = false;
     ctx->fq_bytes = nbytes;

@@ -39,6 +39,7 @@ struct Job {
     mp_job_result res{};
     std::vector<uint32_t> parent;
     bool done = false;
+    bool claimed = false;  // a thread waits for it (a job is waited for once)
 };
 
 // Device bytes of one job in a batch, from its FASTQ size: the FASTQ itself,
@@ -224,8 +225,10 @@ int mp_server_wait(mp_server *s, uint64_t job, mp_job_result *res, uint32_t *par
         auto it = s->jobs.find(job);
         if (it == s->jobs.end()) return MP_EINVAL;
         j = it->second;
+        if (j->claimed) return MP_ESTATE;  // another thread waits for it
+        j->claimed = true;
         s->cv_done.wait(lk, [&] { return j->done; });
-        s->jobs.erase(it);
+        s->jobs.erase(job);  // by id: the map may have changed while this thread waited
     }
     *res = j->res;
     if (parent && !j->parent.empty()) {

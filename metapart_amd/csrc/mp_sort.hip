@@ -595,11 +595,14 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         A.npos = gen->npos;
     }
     int executed = 0;
+    // MP_SORT_STATUS64 forces the 64-bit status words on every pass (tests run
+    // the small inputs through the format the largest inputs need)
+    const bool force64 = getenv("MP_SORT_STATUS64") != nullptr;
     for (int p = 0; p < kPasses; p++) {
         if (!run[p]) continue;
         // status words: 32-bit when every digit's count fits 26 bits; the array is
         // cleared when the format changes or the epoch wraps (4 bits for 32-bit words)
-        const bool s32 = dmax[p] < (1ull << 26);
+        const bool s32 = dmax[p] < (1ull << 26) && !force64;
         ++ctx->epoch;
         if (ctx->epoch == 0 || ctx->status_fmt != (s32 ? 1 : 2) || (s32 && (ctx->epoch & 15) == 0)) {
             MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));

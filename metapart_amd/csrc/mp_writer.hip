@@ -156,6 +156,10 @@ int write_partitions(mp_ctx *ctx, const char *out_prefix, int T, const uint64_t 
     const bool batch = ctx->batch;
     const int F = MP_NPART * T * (batch ? C : 1);
     if (F >= (int)kNoFile) return ctx->fail(MP_EINVAL, "too many output files");
+    // writer items are numbered 2 r and 2 r + 1 in 32 bits
+    if (2 * N > (uint64_t)UINT32_MAX)
+        return ctx->fail(MP_EINVAL, "the writer takes at most 2,147,483,647 reads (%llu loaded)",
+                         (unsigned long long)N);
     if (batch && (T != 1 || ctx->sample_W.size() != (size_t)C * (MP_NPART + 1)))
         return ctx->fail(MP_ESTATE, "batch output needs T = 1 and per-sample plans");
     std::vector<int64_t> il(C, -1);

@@ -57,7 +57,11 @@ def masked_sha(path):
 
 
 def sha(path):
-    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for blk in iter(lambda: f.read(64 << 20), b""):
+            h.update(blk)
+    return h.hexdigest()
 
 
 def hash_outputs(d):
@@ -72,11 +76,26 @@ def hash_outputs(d):
     return out
 
 
-def run_chain(kind, case, d, timeout=600):
+def golden_large():
+    with open(os.path.join(GOLDEN_DIR, "golden_large.json")) as f:
+        return json.load(f)
+
+
+def make_large_input(name, dst):
+    """Writes the input of a workload-size golden case (seeded generator)."""
+    args = golden_large()["inputs"][name]
+    subprocess.run([os.path.join(BIN, "mp_synth")] + args + ["-o", dst], check=True)
+
+
+def run_chain(kind, case, d, timeout=600, input_path=None):
     """Runs genFastqIdx, genHistograms_metapart[min], metapart[min] of `kind`
-    ('oracle' = CPU restatement, 'gpu' = this build's tools) in directory d."""
+    ('oracle' = CPU restatement, 'gpu' = this build's tools) in directory d,
+    on the case's input (or on input_path, hard-linked into d)."""
     os.makedirs(os.path.join(d, "out", "Rank0"), exist_ok=True)
-    make_input(case["input"], os.path.join(d, "in.fastq"))
+    if input_path:
+        os.link(input_path, os.path.join(d, "in.fastq"))
+    else:
+        make_input(case["input"], os.path.join(d, "in.fastq"))
     a = ["0", "0", "1"] if case["mode"] == "il" else ["0", "1", "0"]
     v = case["variant"]
     C, K, S, T = str(case["C"]), str(case["K"]), str(case["S"]), str(case["T"])
@@ -176,3 +195,21 @@ def read_kmeridx(path):
     hdr = [dict(offset=int(r[0:8].view(np.uint64)[0]), readIdExtra=int(r[16]), readId=int(r[20:24].view(np.uint32)[0]),
                 fileNo=int(r[24:28].view(np.int32)[0])) for r in recs]
     return hdr, counts
+
+
+def oracle_sample(job):
+    """One sample through the oracle in a process of its own (for process
+    pools: nothing here touches the GPU).  job = (fastq path, C, K, S, T,
+    out_dir); returns (parent labels as uint32, {file name: sha256})."""
+    path, C, K, S, T, out_dir = job
+    r = oracle_run(path, C, K, S, T, out_dir=out_dir)
+    od = os.path.join(out_dir, "out", "Rank0")
+    files = {f: sha(os.path.join(od, f)) for f in sorted(os.listdir(od))}
+    return r["parent"].astype(np.uint32), files
+
+
+def oracle_samples(jobs, workers=8):
+    """oracle_sample over many jobs in fresh interpreter processes."""
+    import multiprocessing as mpr
+    with mpr.get_context("spawn").Pool(processes=max(1, min(workers, len(jobs)))) as pool:
+        return pool.map(oracle_sample, jobs)

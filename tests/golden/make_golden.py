@@ -4,7 +4,9 @@ writes for a set of seeded inputs.  Run here, where the reference sources
 exist; the JSON is committed and is what tests compare the oracle and the GPU
 tools against.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # small cases -> golden.json
+    python tests/golden/make_golden.py --large    # workload-size cases -> golden_large.json
+    python tests/golden/make_golden.py --viewidx  # adds the viewIdx output hash to golden.json
 
 Inputs are either the reference's own fixture (tests/golden/reads.fastq, a
 copy of data/reads.fastq: 2,500 interleaved 90 bp records) or FASTQ made by
@@ -73,6 +75,25 @@ CASES = [
 ]
 
 
+# Workload-size cases (minutes of reference CPU time each; golden_large.json).
+# The 20 M-read medium workload (200 genomes of 1-5 Mb, GC U(0.3, 0.7), ten
+# 5 kb repeats, 150 bp reads, 1 % errors) at k = 31 with 16 chunks: two scans
+# (64 GB of host memory holds half its 2.4 G tuples at a time), one and two
+# segments, and the 3-minimizer variant with one scan; and the first 2 M reads
+# of the large workload (1000 genomes of 0.5-8 Mb, log-normal(1.5)) with 64
+# chunks and four scans.
+LARGE_INPUTS = {
+    "c2": ["--preset", "2"],
+    "c3slice": ["--preset", "3", "--reads", "2000000"],
+}
+LARGE_CASES = [
+    ("c3slice_k31_S4", "c3slice", 64, 31, 4, 1, "se", ""),
+    ("c2_k31_S2T1", "c2", 16, 31, 2, 1, "se", ""),
+    ("c2_min3_k31", "c2", 16, 31, 1, 1, "se", "min"),
+    ("c2_k31_S2T2", "c2", 16, 31, 2, 2, "se", ""),
+]
+
+
 def masked_sha(path):
     b = bytearray(open(path, "rb").read())
     for r in range(len(b) // REC):
@@ -92,7 +113,8 @@ def make_input(name, dst):
     if name.endswith(".fastq"):
         shutil.copy(os.path.join(HERE, name), dst)
     else:
-        subprocess.run([SYNTH] + SYNTH_INPUTS[name] + ["-o", dst], check=True)
+        args = SYNTH_INPUTS[name] if name in SYNTH_INPUTS else LARGE_INPUTS[name]
+        subprocess.run([SYNTH] + args + ["-o", dst], check=True)
 
 
 def hash_outputs(d, K, variant):
@@ -107,11 +129,17 @@ def hash_outputs(d, K, variant):
     return out
 
 
-def run_reference(case, work):
+def run_reference(case, work, cache=None):
     name, inp, C, K, S, T, mode, variant = case
     d = os.path.join(work, name)
     os.makedirs(os.path.join(d, "out", "Rank0"))
-    make_input(inp, os.path.join(d, "in.fastq"))
+    if cache is not None:  # large inputs are generated once and linked into each case
+        if inp not in cache:
+            cache[inp] = os.path.join(work, inp + ".fastq")
+            make_input(inp, cache[inp])
+        os.link(cache[inp], os.path.join(d, "in.fastq"))
+    else:
+        make_input(inp, os.path.join(d, "in.fastq"))
     a = ["0", "0", "1"] if mode == "il" else ["0", "1", "0"]
     env = dict(os.environ, OMP_NUM_THREADS="1")
     q = dict(cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
@@ -119,10 +147,79 @@ def run_reference(case, work):
     subprocess.run([os.path.join(REF, "genHistograms_metapart" + variant), str(K)] + a + ["in.fastq"], **q)
     subprocess.run([os.path.join(REF, "metapart" + variant), "-o", "./out", "2000000000", str(K), str(S), str(T)] + a +
                    ["in.fastq"], **q)
-    return hash_outputs(d, K, variant)
+    h = hash_outputs(d, K, variant)
+    if cache is not None:
+        shutil.rmtree(d, ignore_errors=True)
+    return h
+
+
+def main_large():
+    work = tempfile.mkdtemp(prefix="golden_large_", dir=os.environ.get("MP_GOLDEN_WORK"))
+    out = os.path.join(HERE, "golden_large.json")
+    res = {"inputs": LARGE_INPUTS, "cases": []}
+    if os.path.exists(out):
+        res = json.load(open(out))
+    done = {c["name"] for c in res["cases"]}
+    cache = {}
+    try:
+        for case in LARGE_CASES:
+            if case[0] in done:
+                continue
+            h = run_reference(case, work, cache)
+            name, inp, C, K, S, T, mode, variant = case
+            res["cases"].append({"name": name, "input": inp, "C": C, "K": K, "S": S, "T": T, "mode": mode,
+                                 "variant": variant, "sha256": h})
+            with open(out, "w") as f:
+                json.dump(res, f, indent=1, sort_keys=True)
+            print(name, len(h), "files", flush=True)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def mask_idx(path):
+    """The .idx with the bytes the reference leaves uninitialised set to zero."""
+    b = bytearray(open(path, "rb").read())
+    for r in range(len(b) // REC):
+        o = r * REC
+        if r == 0:
+            b[o + 16] = 0
+        b[o + 17:o + 20] = b"\0\0\0"
+        b[o + REC - 4:o + REC] = b"\0\0\0\0"
+    open(path, "wb").write(bytes(b))
+
+
+def main_viewidx():
+    """Adds to every case of golden.json the SHA-256 of what the reference's
+    viewIdx prints for the reference's .idx of the case (uninitialised bytes
+    zeroed first, as this build writes them)."""
+    out = os.path.join(HERE, "golden.json")
+    res = json.load(open(out))
+    work = tempfile.mkdtemp(prefix="golden_view_")
+    try:
+        for c in res["cases"]:
+            d = os.path.join(work, c["name"])
+            os.makedirs(d)
+            make_input(c["input"], os.path.join(d, "in.fastq"))
+            a = ["0", "0", "1"] if c["mode"] == "il" else ["0", "1", "0"]
+            subprocess.run([os.path.join(REF, "genFastqIdx"), str(c["C"])] + a + ["in.fastq"], cwd=d,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+            mask_idx(os.path.join(d, "in.fastq.idx"))
+            r = subprocess.run([os.path.join(REF, "viewIdx"), "in.fastq.idx"], cwd=d, stdout=subprocess.PIPE,
+                               check=True)
+            c["viewidx_sha256"] = hashlib.sha256(r.stdout).hexdigest()
+            shutil.rmtree(d)
+            print(c["name"], c["viewidx_sha256"][:16])
+        with open(out, "w") as f:
+            json.dump(res, f, indent=1, sort_keys=True)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
 
 
 def main():
+    if "--large" in sys.argv:
+        return main_large()
+    if "--viewidx" in sys.argv:
+        return main_viewidx()
     if not os.path.exists(os.path.join(REF, "metapart")):
         sys.exit("build the reference first: make -C oracle all")
     work = tempfile.mkdtemp(prefix="golden_")

@@ -63,6 +63,49 @@ def test_index_tool_matches_reference(case, tmp_path):
     assert masked_sha(os.path.join(d, "in.fastq.idx")) == case["sha256"]["idx"]
 
 
+@pytest.mark.parametrize("case", [c for c in golden()["cases"]], ids=[c["name"] for c in golden()["cases"]])
+def test_view_index_tool_matches_reference(case, tmp_path):
+    """viewIdx (viewIdx.c:18-49) over this build's .idx prints what the
+    reference's viewIdx prints over the reference's .idx (whose uninitialised
+    bytes are zeroed, as this build writes them)."""
+    import hashlib
+    d = str(tmp_path)
+    make_input(case["input"], os.path.join(d, "in.fastq"))
+    a = ["0", "0", "1"] if case["mode"] == "il" else ["0", "1", "0"]
+    subprocess.run([os.path.join(BIN, "genFastqIdx"), str(case["C"])] + a + ["in.fastq"], cwd=d, check=True,
+                   stdout=subprocess.DEVNULL)
+    r = subprocess.run([os.path.join(BIN, "viewIdx"), "in.fastq.idx"], cwd=d, check=True, stdout=subprocess.PIPE)
+    assert hashlib.sha256(r.stdout).hexdigest() == case["viewidx_sha256"]
+    lines = r.stdout.decode().splitlines()
+    assert len(lines) == case["C"] + 2 and lines[-1] == "TotalKmers 0"
+
+
+def test_two_file_paired_end_input_is_refused(tmp_path):
+    """Two-file paired-end input is refused with an error by every tool and by
+    the C ABI: the reference gives both mates one read id (metapart.c:527-540),
+    sizes its buffers from histograms that never read the mate file
+    (genHistograms.c reads files[0] only), and its writer drops those reads
+    (metapart.c:1037-1047), so there is no defined result to reproduce."""
+    d = str(tmp_path)
+    buf = mp.synth(0, n_reads=200)
+    for f in ("a_1.fastq", "a_2.fastq"):
+        open(os.path.join(d, f), "wb").write(buf.tobytes())
+    os.makedirs(os.path.join(d, "out", "Rank0"))
+    files = ["a_1.fastq", "a_2.fastq"]
+    for cmd in ([os.path.join(BIN, "genFastqIdx"), "2", "1", "0", "0"] + files,
+                [os.path.join(BIN, "genHistograms_metapart"), "27", "1", "0", "0"] + files,
+                [os.path.join(BIN, "genHistograms_metapartmin"), "27", "1", "0", "0"] + files,
+                [os.path.join(BIN, "metapart"), "-o", "./out", "2000000000", "27", "1", "1", "1", "0", "0"] + files,
+                [os.path.join(BIN, "metapartmin"), "-o", "./out", "2000000000", "27", "1", "1", "1", "0", "0"] + files):
+        r = subprocess.run(cmd, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        assert r.returncode != 0, cmd[0]
+        assert "not supported" in r.stderr or "unsupported" in r.stderr, (cmd[0], r.stderr)
+    assert not os.path.exists(os.path.join(d, "a_1.fastq.idx"))
+    assert os.listdir(os.path.join(d, "out", "Rank0")) == []
+    arr = (ct.c_char_p * 2)(*[os.path.join(d, f).encode() for f in files])
+    assert mp.lib().mp_index_fastq(2, 1, 0, 0, arr, 2) == -1
+
+
 def test_index_buffer_equals_index_file(tmp_path):
     buf = mp.synth(1)
     p = os.path.join(str(tmp_path), "in.fastq")

@@ -17,3 +17,17 @@ def test_gpu_tools_match_reference(case, tmp_path):
     print("gpu golden case", case["name"], flush=True)
     got = run_chain("gpu", case, str(tmp_path), timeout=300)
     assert diff_hashes(got, case["sha256"]) == []
+
+
+STATUS64 = ["c1_T2", "gcrep_k31_S4T2", "mixed_k27_S2T2", "var_k13_S2T2"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", STATUS64)
+def test_gpu_tools_match_reference_with_64bit_status(name, tmp_path, monkeypatch):
+    """The same with the radix passes' 64-bit look-back status words forced."""
+    print("gpu golden case (64-bit status)", name, flush=True)
+    monkeypatch.setenv("MP_SORT_STATUS64", "1")
+    case = next(c for c in CASES if c["name"] == name)
+    got = run_chain("gpu", case, str(tmp_path), timeout=300)
+    assert diff_hashes(got, case["sha256"]) == []

@@ -51,7 +51,12 @@ def test_workload_generators_are_seeded():
     assert a == b and len(a) == 64
     assert all(x["n_reads"] % 2 == 0 and 10_000 <= x["n_reads"] <= 5_000_002 for x in a)
     assert all(20 <= x["n_genomes"] <= 300 and 100 <= x["read_len"] <= 150 for x in a)
+    assert [x["seed"] for x in a] == list(range(100, 164))  # the mix's seeds 100-163
+    assert all(x["genome_len_min"] == 100_000 and x["genome_len_max"] == 2_000_000 for x in a)
     j = mp.job_params(seed=7)
-    assert j == mp.job_params(seed=7) and len(j) == 64
+    assert j == mp.job_params(seed=7) and len(j) == 200
+    assert [x["seed"] for x in j] == list(range(5000, 5200))  # job j: seed 5000 + j
+    assert all(20 <= x["n_genomes"] <= 100 and x["genome_len_min"] == 500_000 and x["genome_len_max"] == 3_000_000
+               for x in j)
     assert all(x["n_reads"] % 2 == 0 and 50_000 <= x["n_reads"] <= 2_000_002 and x["read_len"] == 150 for x in j)
     assert len({x["seed"] for x in j}) == len(j)  # a fresh genome pool per job
