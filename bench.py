@@ -11,16 +11,25 @@ range plan, k-mer tuple enumeration, radix sort, frequency filter, union-find,
 compress, pair step, partition plan and the device-side build of the 16
 partitioned FASTQ streams.  Host file writes are not part of the step.
 
-The same line also reports the 3-minimizer variant on the same reads
-("variants") and the batched form ("batched"): a 64-sample mix (seeded genome
-pool per sample of 20-300 genomes of 0.1-1 Mb, log-uniform 10k-5M reads of
-100-150 bp, 0.5-2 % errors, k = 31) resident in HBM, run once through the
-plain per-sample path (one context, samples one after another, each a full
-histogram/plan/scan/finish/writer run) and once as one batch; the plans of
-every sample must agree between the two.
+The same JSON line also reports:
+  * "stages": per-kernel device time, algorithmic bytes and GB/s against the
+    8 TB/s HBM peak (events on the library's stream);
+  * "tools_wall": the three drop-in tools run end to end on the same reads
+    (FASTQ in the page cache, output files on local disk);
+  * "cpu_baseline": the reference implementation (built from its sources) on
+    the first 2 M reads, all host cores up to 16, median of 3 runs;
+  * "variants": the 3-minimizer variant on the same reads;
+  * "batched": the 64-sample mix (sample i: own pool of 20-300 genomes of
+    0.1-2 Mb seeded 100 + i, log-uniform 10k-5M reads of 100-150 bp, 0.5-2 %
+    errors) resident in HBM, plain per-sample path vs one batch;
+  * "served": 200 Poisson-arriving jobs (job j: seed 5000 + j, 20-100 genomes
+    of 0.5-3 Mb, log-uniform 50k-2M reads of 150 bp) at 0.7 x the plain path's
+    job rate, the request loop vs the plain per-job path (p50/p99 latency);
+  * "out_of_core": the 100 M-read share (1000 genomes of 0.5-8 Mb,
+    log-normal(1.5)), uploaded through pinned staging, scans from the budget.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--reads N] [--no-cpu-baseline]
-                       [--no-min3] [--no-batch] [--batch-samples 64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--reads N] [--no-cpu-baseline] [--no-tools]
+                       [--no-min3] [--no-batch] [--no-serve] [--no-c3]
 """
 from __future__ import annotations
 
@@ -53,10 +62,33 @@ def measured_traffic(reads: int):
     return None
 
 
-def cpu_baseline(sample_reads: int, K: int, cores: int = 1):
-    """The reference implementation (built from its sources into oracle/_ref) on
-    the first sample_reads reads of the workload, one thread: genFastqIdx +
-    genHistograms_metapart + metapart wall time."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: the cores this process may use, at
+    most 16 (the host share of one GPU on these nodes)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(sample_reads: int, K: int, threads: int, runs: int = 3):
+    """The reference implementation (built from its sources into oracle/_ref)
+    on the first sample_reads reads of the workload: genFastqIdx +
+    genHistograms_metapart + metapart wall time with `threads` OpenMP threads,
+    4 chunks per thread, one scan; the median of `runs` runs, and the per-stage
+    line metapart prints (metapart.c:4100) of the median run."""
+    import statistics
     import metapart_amd as mp
     ref = os.path.join(ROOT, "oracle", "_ref")
     if not os.path.exists(os.path.join(ref, "metapart")):
@@ -66,22 +98,76 @@ def cpu_baseline(sample_reads: int, K: int, cores: int = 1):
     try:
         fq = os.path.join(d, "in.fastq")
         with open(fq, "wb") as f:
-            f.write(buf.tobytes())
+            f.write(memoryview(buf.view()))
         buf.free()
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        C = 4 * threads
+        walls, lines = [], []
+        for _ in range(runs):
+            od = os.path.join(d, "out")
+            shutil.rmtree(od, ignore_errors=True)
+            os.makedirs(os.path.join(od, "Rank0"))
+            t0 = time.time()
+            line = ""
+            for cmd in ([os.path.join(ref, "genFastqIdx"), str(C), "0", "1", "0", fq],
+                        [os.path.join(ref, "genHistograms_metapart"), str(K), "0", "1", "0", fq],
+                        [os.path.join(ref, "metapart"), "-o", od, "2000000000", str(K), "1", str(threads),
+                         "0", "1", "0", fq]):
+                r = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                                   timeout=600)
+                if r.returncode != 0:
+                    return None
+                for ln in r.stdout.splitlines():
+                    if "FQRead" in ln:
+                        line = ln.strip()
+            walls.append(time.time() - t0)
+            lines.append(line)
+        med = statistics.median(walls)
+        return {"value": sample_reads / med, "unit": "reads/s", "cores": threads, "kind": "reference",
+                "cpu_model": cpu_model(), "runs_s": walls,
+                "sample": "first %d reads of the same synthetic workload (k=%d, %d chunks, S=1, T=%d, OMP_NUM_THREADS=%d), "
+                          "genFastqIdx+genHistograms_metapart+metapart wall, median of %d runs: %.2f s"
+                          % (sample_reads, K, C, threads, threads, runs, med),
+                "stage_line": lines[walls.index(med)] if med in walls else lines[0]}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def tools_wall(buf, reads: int, K: int, chunks: int):
+    """The drop-in tools end to end, as a user runs them: genFastqIdx,
+    genHistograms_metapart and metapart (S=1, T=1) on the workload's FASTQ file
+    (written first, so it is in the page cache), partitioned files written to
+    local disk.  Each tool is a process of its own (GPU runtime start-up
+    included); the wall time of each and metapart's stage line."""
+    bindir = os.path.join(ROOT, "metapart_amd", "bin")
+    d = tempfile.mkdtemp(prefix="mp_tools_")
+    try:
+        fq = os.path.join(d, "in.fastq")
+        with open(fq, "wb") as f:
+            f.write(memoryview(buf.view()))
         os.makedirs(os.path.join(d, "out", "Rank0"))
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-        t0 = time.time()
-        for cmd in ([os.path.join(ref, "genFastqIdx"), "16", "0", "1", "0", fq],
-                    [os.path.join(ref, "genHistograms_metapart"), str(K), "0", "1", "0", fq],
-                    [os.path.join(ref, "metapart"), "-o", os.path.join(d, "out"), "2000000000", str(K), "1", str(cores),
-                     "0", "1", "0", fq]):
-            r = subprocess.run(cmd, cwd=d, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+        walls, line, reads_line = [], "", ""
+        for cmd in ([os.path.join(bindir, "genFastqIdx"), str(chunks), "0", "1", "0", "in.fastq"],
+                    [os.path.join(bindir, "genHistograms_metapart"), str(K), "0", "1", "0", "in.fastq"],
+                    [os.path.join(bindir, "metapart"), "-o", "./out", "2000000000", str(K), "1", "1", "0", "1", "0",
+                     "in.fastq"]):
+            t0 = time.time()
+            r = subprocess.run(cmd, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=900)
+            walls.append(time.time() - t0)
             if r.returncode != 0:
-                return None
-        dt = time.time() - t0
-        return {"value": sample_reads / dt, "unit": "reads/s", "cores": cores, "kind": "reference",
-                "sample": "first %d reads of the same synthetic workload (k=%d, 16 chunks, S=1, T=%d), "
-                          "genFastqIdx+genHistograms_metapart+metapart wall %.2f s" % (sample_reads, K, cores, dt)}
+                return {"error": "%s failed: %s" % (os.path.basename(cmd[0]), r.stdout[-500:])}
+            for ln in r.stdout.splitlines():
+                if "FQRead" in ln:
+                    line = ln.strip()
+                if "reads/s" in ln:
+                    reads_line = ln.strip()
+        out_bytes = sum(os.path.getsize(os.path.join(d, "out", "Rank0", f))
+                        for f in os.listdir(os.path.join(d, "out", "Rank0")))
+        total = sum(walls)
+        return {"reads_per_s": reads / total, "wall_s": total,
+                "tool_wall_s": {"genFastqIdx": walls[0], "genHistograms_metapart": walls[1], "metapart": walls[2]},
+                "metapart_stage_line": line, "metapart_summary": reads_line, "out_bytes": out_bytes,
+                "setup": "FASTQ in the page cache, %d chunks, S=1, T=1, output files on local disk" % chunks}
     finally:
         shutil.rmtree(d, ignore_errors=True)
 
@@ -94,18 +180,21 @@ def main():
     ap.add_argument("--reads", type=int, default=20_000_000)
     ap.add_argument("--k", type=int, default=31)
     ap.add_argument("--chunks", type=int, default=16)
-    ap.add_argument("--cpu-sample", type=int, default=600_000)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the cores this process may use, at most 16")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tools", action="store_true", help="skip the end-to-end run of the drop-in tools")
     ap.add_argument("--no-min3", action="store_true", help="skip the 3-minimizer variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-sample mix")
     ap.add_argument("--batch-samples", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
-    ap.add_argument("--c3", action="store_true",
-                    help="also run the large out-of-core share (100M reads; needs ~45 GB of host memory)")
+    ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="arrival rate as a fraction of the plain path's job rate at the mean job")
+    ap.add_argument("--no-c3", action="store_true", help="skip the large out-of-core share (100M reads)")
     ap.add_argument("--c3-reads", type=int, default=100_000_000)
-    ap.add_argument("--serve-jobs", type=int, default=64)
-    ap.add_argument("--serve-rate", type=float, default=60.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device copy first (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -117,12 +206,18 @@ def main():
     import metapart_amd as mp
 
     # CPU baseline first, before this process touches the GPU
-    cpu = None if args.no_cpu_baseline else cpu_baseline(min(args.cpu_sample, args.reads), args.k)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(min(args.cpu_sample, args.reads), args.k, args.cpu_threads or cpu_threads(),
+                           args.cpu_runs)
+        print("cpu baseline done", file=sys.stderr, flush=True)
 
     t0 = time.time()
     buf = mp.synth(2, n_reads=args.reads)
     gen_s = time.time() - t0
     host = buf.view()
+    tools = None if args.no_tools else tools_wall(buf, args.reads, args.k, args.chunks)
+    print("tools done", file=sys.stderr, flush=True)
 
     import torch
     dev = torch.device("cuda:0")
@@ -166,10 +261,12 @@ def main():
         results[variant] = {"ms_per_step": 1e3 * dt / args.steps, "reads_per_s": args.reads * args.steps / dt,
                             "scan": sc, "finish": r.finish, "host_stage_s": tstage, "out_bytes": r.out_bytes}
         ctx.close()
+    print("timed steps done", file=sys.stderr, flush=True)
 
     plain = results["plain"]
     sc = plain["scan"]
     n = int(sc["tuples"])
+    N = args.reads
     passes = int(sc["sort_passes"])
     pass_ms = float(sc["ms_sort_pass_avg"])
     per_pass_bytes = 2 * TUPLE_BYTES * n  # read and write 13 B per tuple
@@ -177,14 +274,37 @@ def main():
     # the local pass (when it ran) reads every key and moves only the tuples not
     # yet at their rank: their label and adj are read and the tuple written
     local = float(sc["ms_sort_local"]) > 0
-    local_bytes = n * 8 + int(sc["sort_local_moved"]) * (5 + TUPLE_BYTES) if local else 0
+    moved = int(sc["sort_local_moved"])
+    local_bytes = n * 8 + moved * (5 + TUPLE_BYTES) if local else 0
     full = passes - (1 if local else 0)  # radix passes over the whole array
-    if float(sc["ms_sort_gen"]) > 0:  # the first pass generates the tuples: it writes them and reads the reads
+    gen_ms = float(sc["ms_sort_gen"])
+    bases = N * 150
+    gen_bytes = n * TUPLE_BYTES + bases * 3 // 8 + bases  # tuples written; 2-bit + 'N' streams and sequence bytes read
+    if gen_ms > 0:  # the first pass generates the tuples: it writes them and reads the reads
         sort_bytes = n * TUPLE_BYTES + n * 2 * (full - 1) * TUPLE_BYTES + local_bytes
     else:  # enumerated tuples: one histogram read (k-mers), then every pass reads and writes them
         sort_bytes = n * (8 + 2 * full * TUPLE_BYTES) + local_bytes
     sort_ms = float(sc["ms_sort"])
     traffic = measured_traffic(args.reads)
+
+    def stage(ms, nbytes, what):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"ms": ms, "algorithmic_bytes": int(nbytes), "GB_per_s": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+                "bytes": what}
+
+    stages = {
+        "sort_generating_pass": stage(gen_ms, gen_bytes, "13 B per tuple written; 0.375 B per base of 2-bit and 'N' "
+                                                          "words and 1 B per base of FASTQ sequence read"),
+        "sort_scatter_pass": stage(pass_ms, per_pass_bytes, "13 B per tuple read and written (one pass, the kernel "
+                                                            "alone)"),
+        "sort_digit_counts": stage(float(sc["ms_sort_count"]), 0, "all counted passes: per-tile digit counts and "
+                                                                  "offsets (no algorithmic bytes)"),
+        "sort_local_pass": stage(float(sc["ms_sort_local"]), local_bytes, "8 B key per tuple read; 5 B read and 13 B "
+                                                                           "written per tuple moved"),
+        "filter": stage(float(sc["ms_filter"]), 9 * n, "8 B key and 1 B adj per tuple read"),
+        "union": stage(float(sc["ms_union"]), n // 4 + 4 * n, "run bitmaps and the labels (random parent "
+                                                              "accesses not counted)"),
+    }
     out = {
         "metric": "reads partitioned/s at k=31 on 1 GPU (plain path); % HBM peak in sort",
         "value": plain["reads_per_s"],
@@ -209,10 +329,12 @@ def main():
         "sort": {"passes": passes, "digit_bits": 8, "tuple_bytes": TUPLE_BYTES, "algorithmic_bytes": sort_bytes,
                  "ms": sort_ms, "hbm_frac": sort_bytes / (sort_ms * 1e-3) / 1e12 / (HBM_PEAK_GBS / 1e3)
                  if sort_ms > 0 else 0.0},
-        "stages_ms": {k: sc[k] for k in ("ms_enumerate", "ms_sort", "ms_sort_gen", "ms_sort_local", "ms_filter",
-                                         "ms_union")},
+        "stages": stages,
+        "stages_ms": {k: sc[k] for k in ("ms_enumerate", "ms_sort", "ms_sort_gen", "ms_sort_local", "ms_sort_count",
+                                         "ms_filter", "ms_union")},
         "finish": plain["finish"],
         "host_stage_s": plain["host_stage_s"],
+        "tools_wall": tools,
         "cpu_baseline": cpu,
         "generate_s": gen_s,
     }
@@ -223,15 +345,19 @@ def main():
                                     "host_stage_s": m3["host_stage_s"],
                                     "stages_ms": {k: m3["scan"][k] for k in ("ms_enumerate", "ms_sort", "ms_filter",
                                                                              "ms_union")}}}
+    del d_fq
+    buf.free()
+    torch.cuda.empty_cache()
     if not args.no_batch:
-        del d_fq
-        buf.free()
-        torch.cuda.empty_cache()
         out["batched"] = batch_section(args, mp, torch, dev)
+        torch.cuda.empty_cache()
+        print("batched done", file=sys.stderr, flush=True)
     if not args.no_serve:
         out["served"] = served_section(args, mp)
-    if args.c3:
+        print("served done", file=sys.stderr, flush=True)
+    if not args.no_c3:
         out["out_of_core"] = c3_section(args, mp, torch)
+        print("out-of-core done", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
@@ -277,23 +403,38 @@ def _pct(xs, q):
 
 
 def served_section(args, mp):
-    """Poisson-arriving jobs from host memory: the request loop (2 engines,
-    continuous batching under an HBM budget) against the plain per-job path
-    (one context, jobs one at a time in arrival order), same arrivals."""
+    """Poisson-arriving partition jobs (job j: seed 5000 + j, its own pool of
+    20-100 genomes of 0.5-3 Mb, log-uniform 50k-2M reads of 150 bp, 1 % errors)
+    from host memory, at a rate of serve_load x the plain path's job rate at
+    the mean job (529 k reads): the request loop (2 engines, HBM-budget
+    admission, continuous batching) against the plain per-job path (one
+    context, jobs one at a time in arrival order), same arrivals."""
     import numpy as np
     params = mp.job_params(n_jobs=args.serve_jobs, seed=7)
     t0 = time.time()
-    bufs = [mp.synth(0, **p) for p in params]
+    bufs = mp.synth_many(params, threads=8)
     gen_s = time.time() - t0
     reads = [p["n_reads"] for p in params]
-    rng = np.random.default_rng(11)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(bufs)))
     warm = mp.synth(0, n_reads=20000, seed=5, read_len=150)
-
-    # plain per-job path
+    # the plain path's job rate at the mean job
+    mean_job = mp.synth(0, seed=4999, n_genomes=60, genome_len_min=500_000, genome_len_max=3_000_000, abund_sigma=1.0,
+                        n_reads=529_000, read_len=150, err_rate=0.01)
     ctx = mp.Context(k=args.k)
     ctx.load(warm, [0, warm.nbytes])
     mp.partition(ctx, S=1, T=1, build_output=False)
+    ts = []
+    for _ in range(4):
+        a = time.perf_counter()
+        ctx.load(mean_job, [0, mean_job.nbytes])
+        mp.partition(ctx, S=1, T=1, build_output=False)
+        ts.append(time.perf_counter() - a)
+    mean_job.free()
+    capacity = 1.0 / float(np.median(ts))
+    rate = args.serve_load * capacity
+    rng = np.random.default_rng(11)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(bufs)))
+
+    # plain per-job path
     lat_plain = []
     start = time.perf_counter()
     for i, b in enumerate(bufs):
@@ -322,10 +463,13 @@ def served_section(args, mp):
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
     for b in bufs:
         b.free()
-    return {"workload": "%d jobs, log-uniform 50k-2M SE reads of 150 bp, 1%% error, k=%d, fresh genome pool per job "
-                        "(20-100 genomes of 0.1-1 Mb), FASTQ in host memory, Poisson arrivals at %.0f jobs/s"
-                        % (len(bufs), args.k, args.serve_rate),
-            "jobs": len(bufs), "reads": int(sum(reads)), "rate_jobs_per_s": args.serve_rate,
+    warm.free()
+    return {"workload": "%d jobs, log-uniform 50k-2M SE reads of 150 bp, 1%% error, k=%d, job j: fresh pool of 20-100 "
+                        "genomes of 0.5-3 Mb seeded 5000+j; FASTQ in host memory; Poisson arrivals at %.1f jobs/s = "
+                        "%.2f x the plain path's %.1f jobs/s at a 529k-read job"
+                        % (len(bufs), args.k, rate, args.serve_load, capacity),
+            "jobs": len(bufs), "reads": int(sum(reads)), "rate_jobs_per_s": rate,
+            "plain_capacity_jobs_per_s": capacity, "mean_job_ms": 1e3 / capacity,
             "served": {"p50_ms": 1e3 * _pct(lat, 50), "p99_ms": 1e3 * _pct(lat, 99),
                        "reads_per_s": sum(reads) / span, "engines": 2,
                        "mean_batch_jobs": float(np.mean([r["batch_jobs"] for r in res]))},
@@ -339,7 +483,7 @@ def batch_section(args, mp, torch, dev):
     import numpy as np
     params = mp.mix_params(n_samples=args.batch_samples, seed=4)
     t0 = time.time()
-    bufs = [mp.synth(0, **p) for p in params]
+    bufs = mp.synth_many(params, threads=8)
     gen_s = time.time() - t0
     sizes = [b.nbytes for b in bufs]
     ofs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
@@ -395,8 +539,9 @@ def batch_section(args, mp, torch, dev):
     batch_s = (time.perf_counter() - t0) / args.batch_steps
     bctx.close()
     equal = all(np.array_equal(w, r["W"]) for w, r in zip(Wp, res))
-    return {"workload": "%d samples, own genome pools (20-300 genomes of 0.1-1 Mb), log-uniform 10k-5M SE reads of "
-                        "100-150 bp, 0.5-2%% error, k=%d, one chunk/scan/segment per sample" % (len(sizes), args.k),
+    return {"workload": "%d samples, sample i: own pool of 20-300 genomes of 0.1-2 Mb (GC U(0.3,0.7)) seeded 100+i, "
+                        "log-uniform 10k-5M SE reads of 100-150 bp, 0.5-2%% error, k=%d, one chunk/scan/segment per "
+                        "sample" % (len(sizes), args.k),
             "samples": len(sizes), "reads": reads, "tuples": int(st.tuples),
             "plain_reads_per_s": reads / plain_s, "batched_reads_per_s": reads / batch_s,
             "speedup": plain_s / batch_s, "ms_plain": 1e3 * plain_s, "ms_batched": 1e3 * batch_s,

@@ -84,13 +84,14 @@ typedef struct {
     int quirk_segments;       /* segments whose first run joined tuple 0's label */
     int sort_passes;          /* radix passes executed */
     float ms_enumerate, ms_sort, ms_filter, ms_union; /* device time */
-    float ms_sort_pass_avg;   /* mean duration of one radix scatter pass */
+    float ms_sort_pass_avg;   /* mean duration of one radix scatter kernel (passes after the first) */
     float ms_sort_hist;       /* the digit-histogram read before the passes */
     float ms_sort_local;      /* the local pass ranking tuples inside high-bit groups (0: all passes global) */
     uint64_t sort_long_groups;/* groups too long for the local pass, sorted by their own global passes */
     float ms_sort_gen;        /* the first radix pass, generating the tuples from the reads (0: enumerated) */
     int parts;                /* 0: one pass; else the scan did not fit the budget and went twice over this many bucket parts */
     uint64_t sort_local_moved;/* tuples the in-place local pass moved (the others were already at their rank) */
+    float ms_sort_count;      /* digit counts and offsets of the passes that take them from a count (not look-back) */
 } mp_scan_stats;
 
 typedef struct {

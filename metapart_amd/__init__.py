@@ -44,7 +44,7 @@ class ScanStats(ct.Structure):
                 ("ms_enumerate", ct.c_float), ("ms_sort", ct.c_float), ("ms_filter", ct.c_float),
                 ("ms_union", ct.c_float), ("ms_sort_pass_avg", ct.c_float), ("ms_sort_hist", ct.c_float),
                 ("ms_sort_local", ct.c_float), ("sort_long_groups", ct.c_uint64), ("ms_sort_gen", ct.c_float),
-                ("parts", ct.c_int), ("sort_local_moved", ct.c_uint64)]
+                ("parts", ct.c_int), ("sort_local_moved", ct.c_uint64), ("ms_sort_count", ct.c_float)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

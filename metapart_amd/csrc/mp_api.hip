@@ -47,6 +47,11 @@ int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
             delete ctx;
             return MP_EDEVICE;
         }
+    for (auto &e : ctx->cev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return MP_EDEVICE;
+        }
     size_t fre = 0, tot = 0;
     (void)hipMemGetInfo(&fre, &tot);
     ctx->budget = cfg->hbm_budget ? cfg->hbm_budget : (uint64_t)(fre * 0.9);
@@ -59,16 +64,12 @@ int mp_ctx_destroy(mp_ctx *ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevArray *arrs[] = {&ctx->fq, &ctx->rec_off, &ctx->seq_off, &ctx->seq_len, &ctx->base_off, &ctx->pos_off,
-                        &ctx->packed, &ctx->nmask, &ctx->parent, &ctx->bad, &ctx->filtered, &ctx->sizes,
-                        &ctx->gparent, &ctx->gsize, &ctx->pairc, &ctx->scanst, &ctx->hpart, &ctx->khist, &ctx->labsnap, &ctx->runbits, &ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
-                        &ctx->zbits, &ctx->wg_cnt, &ctx->wg_off, &ctx->status, &ctx->scratch, &ctx->hist,
-                        &ctx->out, &ctx->items[0], &ctx->items[1], &ctx->ilens[0], &ctx->ilens[1],
-                        &ctx->iaux[0], &ctx->iaux[1]};
-    for (DevArray *a : arrs) release(ctx, *a);
+    ctx->for_each_array([&](DevArray &a) { release(ctx, a); });
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->pev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->cev)
         if (e) (void)hipEventDestroy(e);
     for (int b = 0; b < 2; b++) {
         if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
@@ -241,9 +242,9 @@ int mp_load_reads_device(mp_ctx *ctx, const void *d_fq, uint64_t nbytes, const u
 }
 
 // Device bytes a scan holds per tuple: two tuple buffers (13 B each), the
-// zeroed-run and run bitmaps, the look-back status, and the growth slack of
-// the buffers.
-constexpr uint64_t kScanBytesPerTuple = 29;
+// zeroed-run and run bitmaps, the look-back status, the counted passes'
+// digit bytes and count tables, and the growth slack of the buffers.
+constexpr uint64_t kScanBytesPerTuple = 31;
 
 // Tuples a scan can hold within the budget: the bytes not held by anything
 // else, less a label snapshot and the room the pair step and the writer will
@@ -253,6 +254,7 @@ static uint64_t scan_tuple_room(mp_ctx *ctx)
     const uint64_t N = ctx->N;
     const uint64_t tuple_held = ctx->keys[0].bytes + ctx->keys[1].bytes + ctx->lab[0].bytes + ctx->lab[1].bytes +
                                 ctx->adj[0].bytes + ctx->adj[1].bytes + ctx->zbits.bytes + ctx->runbits.bytes +
+                                ctx->dig.bytes + ctx->rts.bytes +
                                 ctx->status.bytes;
     const uint64_t other = ctx->held - tuple_held;
     const uint64_t label_bytes = ctx->labsnap.bytes ? 0 : N * sizeof(uint32_t) + 64;
@@ -299,7 +301,7 @@ static int scan_two_sweeps(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const u
     // histogram's staging): let each part size them
     MP_HIP(hipStreamSynchronize(ctx->stream));
     for (DevArray *a : {&ctx->keys[0], &ctx->keys[1], &ctx->lab[0], &ctx->lab[1], &ctx->adj[0], &ctx->adj[1],
-                        &ctx->zbits, &ctx->runbits})
+                        &ctx->zbits, &ctx->runbits, &ctx->dig, &ctx->rts})
         release(ctx, *a);
     MP_TRY(snapshot_labels(ctx, s));
     cap = std::min(cap, scan_tuple_room(ctx));  // the counts and the snapshot took their share
@@ -398,6 +400,7 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     st.sort_long_groups = ctx->local_long;
     st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen ? ctx->sort_gen_ms : 0.f;
+    st.ms_sort_count = ctx->sort_count_ms;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     if (stats) *stats = st;
@@ -603,6 +606,7 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     st.sort_long_groups = ctx->local_long;
     st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen_ms;
+    st.ms_sort_count = ctx->sort_count_ms;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     ctx->last_n = n;

@@ -97,13 +97,19 @@ struct mp_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8]{};
     hipEvent_t pev[24]{};  // per radix pass (timing of the dominant kernel)
+    hipEvent_t cev[12]{};  // per counted radix pass: before its digit count
     float sort_pass_avg = 0, sort_hist_ms = 0;
+    float sort_count_ms = 0;   // digit counts and offsets of the counted passes of the last sort
     float sort_gen_ms = 0;     // the generating radix pass of the last sort (0: none)
     float local_ms = 0;        // local rank pass of the last sort
     uint64_t local_long = 0;   // groups it handed to the long-group kernel
     uint64_t local_moved = 0;  // tuples it moved (the rest stayed in place)
     std::string err;
     uint64_t budget = 0, held = 0;
+    // arena mode (the allocation-free entry point): every device buffer is
+    // carved from [arena, arena + arena_bytes), nothing is allocated or freed
+    uint8_t *arena = nullptr;
+    uint64_t arena_bytes = 0, arena_used = 0;
     uint16_t epoch = 0;  // look-back status epoch of the radix sort
     int status_fmt = 0;  // 1: 32-bit status words, 2: 64-bit (the array is cleared on a change)
 
@@ -124,6 +130,7 @@ struct mp_ctx {
     // scan buffers (ping-pong)
     mp::DevArray keys[2], lab[2], adj[2], zbits;
     mp::DevArray runbits;  // filter -> union: starts of runs to join, starts of all runs
+    mp::DevArray dig, rts; // counted radix passes: next pass's digit per tuple; per-tile counts and offsets
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
@@ -153,6 +160,17 @@ struct mp_ctx {
     std::vector<uint64_t> sample_largest;  // largest component per sample
     std::vector<uint64_t> sample_tuples;   // tuple offsets of the samples (C+1)
 
+    // every device buffer the ctx holds
+    template <typename F> void for_each_array(F f)
+    {
+        mp::DevArray *arrs[] = {&fq, &rec_off, &seq_off, &seq_len, &base_off, &pos_off, &packed, &nmask, &parent, &bad,
+                                &filtered, &sizes, &gparent, &gsize, &pairc, &scanst, &hpart, &khist, &labsnap,
+                                &runbits, &dig, &rts, &keys[0], &keys[1], &lab[0], &lab[1], &adj[0], &adj[1], &zbits, &wg_cnt,
+                                &wg_off, &status, &scratch, &hist, &out, &items[0], &items[1], &ilens[0], &ilens[1],
+                                &iaux[0], &iaux[1]};
+        for (mp::DevArray *a : arrs) f(*a);
+    }
+
     int fail(int code, const char *fmt, ...)
     {
         char buf[1024];
@@ -181,6 +199,10 @@ namespace mp {
 // Grow-only device allocation charged against the ctx budget.
 int ensure(mp_ctx *ctx, DevArray &a, uint64_t bytes);
 void release(mp_ctx *ctx, DevArray &a);
+// Stream-ordered temporary device memory (hipMallocAsync / hipFreeAsync, or
+// the ctx's arena in arena mode).
+int tmp_alloc(mp_ctx *ctx, void **p, uint64_t bytes);
+int tmp_free(mp_ctx *ctx, void *p);
 
 // Exclusive scan of n u32 or u64 values into out (n+1 entries: out[n] = total).
 int scan_u32(mp_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);

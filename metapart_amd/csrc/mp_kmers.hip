@@ -246,13 +246,13 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
             pieces.push_back({b + j * kHSlab, std::min(e, b + (j + 1) * kHSlab), (uint32_t)p, n > 1 ? 1u : 0u});
     }
     HPiece *d_pieces = nullptr;
-    MP_HIP(hipMallocAsync((void **)&d_pieces, pieces.size() * sizeof(HPiece), ctx->stream));
+    MP_TRY(tmp_alloc(ctx, (void **)&d_pieces, pieces.size() * sizeof(HPiece)));
     MP_HIP(hipMemcpyAsync(d_pieces, pieces.data(), pieces.size() * sizeof(HPiece), hipMemcpyHostToDevice,
                           ctx->stream));
     hipLaunchKernelGGL(hist_count_kernel, dim3((unsigned)pieces.size()), dim3(1024), 0, ctx->stream,
                        (const uint16_t *)H.items, (const HPiece *)d_pieces, d_counts);
     MP_HIP(hipGetLastError());
-    MP_HIP(hipFreeAsync(d_pieces, ctx->stream));
+    MP_TRY(tmp_free(ctx, d_pieces));
     // the host vectors above must outlive the copies
     MP_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -467,7 +467,7 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
 {
     KParams P = base_params(ctx);
     uint64_t *d_ids = nullptr;
-    MP_HIP(hipMallocAsync((void **)&d_ids, sizeof(uint64_t) * (C + 1), ctx->stream));
+    MP_TRY(tmp_alloc(ctx, (void **)&d_ids, sizeof(uint64_t) * (C + 1)));
     MP_HIP(hipMemcpyAsync(d_ids, chunk_first_id, sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice, ctx->stream));
     MP_TRY(fill_u32(ctx, d_counts, 0, (uint64_t)C * (MP_NBUCKETS + 2)));
     P.counts = d_counts;
@@ -480,7 +480,7 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
     } else {
         MP_TRY(count_plain_partitioned(ctx, P, chunk_first_id, C, d_counts));
     }
-    MP_HIP(hipFreeAsync(d_ids, ctx->stream));
+    MP_TRY(tmp_free(ctx, d_ids));
     return 0;
 }
 

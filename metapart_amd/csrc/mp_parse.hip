@@ -259,7 +259,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     MP_TRY(ensure(ctx, ctx->scratch, 0));
     {
         uint64_t *d_cofs = nullptr, *d_ids = nullptr;
-        MP_HIP(hipMallocAsync((void **)&d_cofs, sizeof(uint64_t) * 2 * (C + 1), ctx->stream));
+        MP_TRY(tmp_alloc(ctx, (void **)&d_cofs, sizeof(uint64_t) * 2 * (C + 1)));
         d_ids = d_cofs + (C + 1);
         MP_HIP(hipMemcpyAsync(d_cofs, chunk_ofs, sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice, ctx->stream));
         MP_TRY(fill_u32(ctx, first_bad, 0xFFFFFFFFu, 2));
@@ -268,7 +268,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
         std::vector<uint64_t> ids(C + 1);
         MP_HIP(hipMemcpyAsync(ids.data(), d_ids, sizeof(uint64_t) * (C + 1), hipMemcpyDeviceToHost, ctx->stream));
         MP_HIP(hipMemcpyAsync(&bad, first_bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
-        MP_HIP(hipFreeAsync(d_cofs, ctx->stream));
+        MP_TRY(tmp_free(ctx, d_cofs));
         MP_HIP(hipStreamSynchronize(ctx->stream));
         if (bad != ~0ull) return ctx->fail(MP_EINVAL, "chunk %llu does not start at a FASTQ record", bad - (1ull << 62));
         if (ids[C] != N) return ctx->fail(MP_EINVAL, "chunks do not end at the last record");
@@ -294,11 +294,11 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
     MP_TRY(ensure(ctx, ctx->nmask, nthreads * sizeof(uint64_t)));
     uint64_t *first = nullptr;
     const uint64_t nb = (tb + kGBlock - 1) / kGBlock;
-    MP_HIP(hipMallocAsync((void **)&first, (nb + 1) * sizeof(uint64_t), ctx->stream));
+    MP_TRY(tmp_alloc(ctx, (void **)&first, (nb + 1) * sizeof(uint64_t)));
     const int rc = launch_gather(ctx, fq, ctx->fq_bytes, (const uint64_t *)ctx->base_off.p, N, tb, nthreads * 64, first,
                                  SeqSrc{(const uint64_t *)ctx->seq_off.p},
                                  PackSink{(uint32_t *)ctx->packed.p, (uint16_t *)ctx->nmask.p});
-    MP_HIP(hipFreeAsync(first, ctx->stream));
+    MP_TRY(tmp_free(ctx, first));
     MP_TRY(rc);
     MP_HIP(hipStreamSynchronize(ctx->stream));
     return 0;

@@ -4,10 +4,43 @@
 
 namespace mp {
 
+// a region of the ctx's arena (arena mode: nothing is allocated or freed;
+// the arena is reset as a whole at the start of each call that uses it)
+static int arena_carve(mp_ctx *ctx, void **p, uint64_t bytes)
+{
+    const uint64_t off = (ctx->arena_used + 255) & ~255ull;
+    if (off + bytes > ctx->arena_bytes)
+        return ctx->fail(MP_ENOMEM, "the input needs more than the %llu-byte fixed scratch (%llu bytes in use, %llu "
+                                    "more asked)", (unsigned long long)ctx->arena_bytes,
+                         (unsigned long long)ctx->arena_used, (unsigned long long)bytes);
+    *p = ctx->arena + off;
+    ctx->arena_used = off + bytes;
+    return 0;
+}
+
+int tmp_alloc(mp_ctx *ctx, void **p, uint64_t bytes)
+{
+    if (ctx->arena) return arena_carve(ctx, p, bytes);
+    MP_HIP(hipMallocAsync(p, bytes, ctx->stream));
+    return 0;
+}
+
+int tmp_free(mp_ctx *ctx, void *p)
+{
+    if (ctx->arena || !p) return 0;
+    MP_HIP(hipFreeAsync(p, ctx->stream));
+    return 0;
+}
+
 int ensure(mp_ctx *ctx, DevArray &a, uint64_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (a.bytes >= bytes) return 0;
+    if (ctx->arena) {  // the old region stays carved until the arena is reset
+        MP_TRY(arena_carve(ctx, &a.p, bytes));
+        a.bytes = bytes;
+        return 0;
+    }
     uint64_t want = bytes + (bytes >> 4);  // 6 % slack so slowly growing runs do not reallocate each time
     // a buffer that grows again at least doubles while a quarter of the budget
     // stays free: a stream of ever larger inputs (a served job mix) then pays a
@@ -41,7 +74,7 @@ int ensure(mp_ctx *ctx, DevArray &a, uint64_t bytes)
 
 void release(mp_ctx *ctx, DevArray &a)
 {
-    if (a.p) {
+    if (a.p && !ctx->arena) {
         (void)hipFree(a.p);
         ctx->held -= a.bytes;
     }

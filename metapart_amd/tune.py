@@ -17,7 +17,7 @@ import numpy as np
 from . import NBUCKETS, Context, MPError, partition, plan_ranges, synth
 
 TUPLE_BYTES = 13           # k-mer u64 + label u32 + adj u8
-PER_TUPLE_DEVICE = 29      # as the library plans a scan: ping-pong buffers, bitmaps, status, growth slack
+PER_TUPLE_DEVICE = 31      # as the library plans a scan: ping-pong buffers, bitmaps, status, digit bytes, slack
 PER_READ_DEVICE = 74       # record table, labels, flags, sizes, a scan's label snapshot
 PER_READ_LATER = 128       # pair-step arrays and writer items, as the library keeps room for them
 TILE_SHAPES = (0, 8192, 4096, 6144, 3072)  # 0: 1024 lanes x 6 tuples

@@ -28,6 +28,7 @@ def test_gpu_tools_match_reference_with_64bit_status(name, tmp_path, monkeypatch
     """The same with the radix passes' 64-bit look-back status words forced."""
     print("gpu golden case (64-bit status)", name, flush=True)
     monkeypatch.setenv("MP_SORT_STATUS64", "1")
+    monkeypatch.setenv("MP_SORT_LOOKBACK", "1")  # every pass by look-back, none from digit counts
     case = next(c for c in CASES if c["name"] == name)
     got = run_chain("gpu", case, str(tmp_path), timeout=300)
     assert diff_hashes(got, case["sha256"]) == []

@@ -45,9 +45,10 @@ def test_stages_match_oracle(case, tmp_path):
     _check_stages(case, tmp_path)
 
 
-# The radix passes switch to 64-bit look-back status words when a digit of a
-# pass holds 2^26 tuples or more (the 20 M-read workload's top pass does);
-# MP_SORT_STATUS64 forces that format, so it meets the oracle on small inputs.
+# The radix passes that find their offsets by look-back (the generating pass;
+# every pass with MP_SORT_LOOKBACK) switch to 64-bit status words when a digit
+# of a pass holds 2^26 tuples or more; MP_SORT_STATUS64 forces that format, so
+# it meets the oracle on small inputs.
 STATUS64 = ["plain_k31_T2", "plain_k31_S3_scan2", "varlen_k25_T3", "pairs_ordered", "deep_repeat"]
 
 
@@ -56,6 +57,7 @@ STATUS64 = ["plain_k31_T2", "plain_k31_S3_scan2", "varlen_k25_T3", "pairs_ordere
 def test_stages_match_oracle_with_64bit_status(name, tmp_path, monkeypatch):
     print("gpu stage case (64-bit status)", name, flush=True)
     monkeypatch.setenv("MP_SORT_STATUS64", "1")
+    monkeypatch.setenv("MP_SORT_LOOKBACK", "1")  # every pass by look-back, none from digit counts
     _check_stages(next(c for c in CASES if c[0] == name), tmp_path)
 
 
