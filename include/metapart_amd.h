@@ -117,6 +117,26 @@ uint64_t mp_ctx_device_bytes(const mp_ctx *ctx);
 int mp_count_chunks(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs, int C,
                     uint32_t *counts, uint64_t *reads_per_chunk, uint64_t *total_kmers);
 
+/* The same histograms without allocating: no counterpart in the reference;
+ * for callers that cannot allocate device memory (request loops, harnesses
+ * that forbid hipMalloc).  The FASTQ bytes are already in device memory
+ * (16-byte aligned, on `device`); counts, reads_per_chunk and total_kmers are
+ * host buffers as in mp_count_chunks, with k and min3 as in mp_cfg.  Every
+ * device buffer comes from one fixed array of MP_NOALLOC_SCRATCH_BYTES in
+ * device memory, which each GPU the library runs on holds until the process
+ * ends; the first call on a device also creates the stream and events the
+ * calls use.  No call allocates or frees device or pinned memory.  Inputs
+ * above MP_NOALLOC_MAX_FASTQ bytes or MP_NOALLOC_MAX_CHUNKS chunks return
+ * MP_EINVAL, and an input whose records need more than the array returns
+ * MP_ENOMEM.  Calls share the device's array: calls that overlap on one GPU
+ * race (serialise them). */
+#define MP_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define MP_NOALLOC_MAX_FASTQ (32ull << 20)
+#define MP_NOALLOC_MAX_CHUNKS 8
+int mp_count_chunks_noalloc(int device, int k, int min3, const void *d_fq, uint64_t nbytes,
+                            const uint64_t *chunk_ofs, int C, uint32_t *counts, uint64_t *reads_per_chunk,
+                            uint64_t *total_kmers);
+
 /* Uploads the FASTQ bytes of a run (C chunks as above) and parses every
  * record; chunk_first_id[c] is the read id of chunk c's first record
  * (KmerIdx readId/readIdExtra), chunk_first_id[C] the read count.

@@ -127,6 +127,7 @@ def lib():
             "mp_server_wait": (i, [vp, u64, ct.POINTER(JobResult), vp, u64]),
             "mp_server_clock": (ct.c_double, [vp]),
             "mp_server_destroy": (i, [vp]),
+            "mp_count_chunks_noalloc": (i, [i, i, i, vp, u64, P64, i, vp, P64, P64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -142,7 +143,7 @@ SYMBOLS = ["mp_index_fastq", "mp_index_buffer", "mp_plan_ranges", "mp_ctx_create
            "mp_load_reads", "mp_load_reads_device", "mp_scan", "mp_finish", "mp_get_parent", "mp_get_bad", "mp_dump_tuples",
            "mp_write_partitions", "mp_batch_load", "mp_batch_load_device", "mp_batch_partition", "mp_batch_get_parent",
            "mp_batch_write", "mp_server_create", "mp_server_submit", "mp_server_wait", "mp_server_clock",
-           "mp_server_destroy"]
+           "mp_server_destroy", "mp_count_chunks_noalloc"]
 
 
 def _u64(a) -> np.ndarray:

@@ -32,7 +32,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SRCS = ["mp_util.hip", "mp_parse.hip", "mp_kmers.hip", "mp_sort.hip", "mp_graph.hip", "mp_writer.hip",
-            "mp_api.hip"]
+            "mp_api.hip", "mp_noalloc.hip"]
 HOST_SRCS = ["mp_host.cpp", "mp_server.cpp"]
 
 
