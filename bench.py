@@ -189,6 +189,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 64-sample mix")
     ap.add_argument("--batch-samples", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--no-graph", action="store_true", help="launch the batch's radix passes one by one")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-load", type=float, default=0.7,
@@ -522,7 +523,8 @@ def batch_section(args, mp, torch, dev):
     del da
     torch.cuda.empty_cache()
 
-    bctx = mp.Context(k=args.k)
+    # the batch's pass chain captured in a HIP graph by the warm-up step, replayed by the timed ones
+    bctx = mp.Context(k=args.k, sort_graph=not args.no_graph)
 
     def batch_step():
         bctx.batch_load_device(base, total, ofs)
@@ -548,6 +550,7 @@ def batch_section(args, mp, torch, dev):
             "plans_equal": bool(equal), "batch_stages_ms": {k: getattr(st, k) for k in
                                                             ("ms_enumerate", "ms_sort", "ms_filter", "ms_union")},
             "batch_sort_passes": st.sort_passes, "batch_sort_pass_ms": st.ms_sort_pass_avg,
+            "sort_graph": {0: "launched pass by pass", 1: "captured", 2: "replayed"}.get(st.sort_graph),
             "generate_s": gen_s}
 
 

@@ -74,6 +74,8 @@ typedef struct {
     uint64_t hbm_budget;  /* device bytes this ctx may hold; 0 = 90 % of free memory */
     int sort_digit_bits;  /* radix digit width: 0 or 8 (the only width; anything else is MP_EINVAL) */
     int sort_tile;        /* tuples per radix tile: 0 (1024 lanes x 6), 3072, 4096, 6144 (512 lanes) or 8192 */
+    int sort_graph;       /* 1: a radix pass chain of counted passes only (a batch's) is captured in a HIP graph
+                             and replayed while the sort's shape and buffers repeat; 0: launched one by one */
 } mp_cfg;
 
 typedef struct {
@@ -92,6 +94,7 @@ typedef struct {
     int parts;                /* 0: one pass; else the scan did not fit the budget and went twice over this many bucket parts */
     uint64_t sort_local_moved;/* tuples the in-place local pass moved (the others were already at their rank) */
     float ms_sort_count;      /* digit counts and offsets of the passes that take them from a count (not look-back) */
+    int sort_graph;           /* the radix pass chain: 0 launched pass by pass, 1 captured in a graph now, 2 replayed */
 } mp_scan_stats;
 
 typedef struct {

@@ -35,7 +35,7 @@ class MPError(RuntimeError):
 
 class Cfg(ct.Structure):
     _fields_ = [("device", ct.c_int), ("k", ct.c_int), ("min3", ct.c_int), ("hbm_budget", ct.c_uint64),
-                ("sort_digit_bits", ct.c_int), ("sort_tile", ct.c_int)]
+                ("sort_digit_bits", ct.c_int), ("sort_tile", ct.c_int), ("sort_graph", ct.c_int)]
 
 
 class ScanStats(ct.Structure):
@@ -44,7 +44,8 @@ class ScanStats(ct.Structure):
                 ("ms_enumerate", ct.c_float), ("ms_sort", ct.c_float), ("ms_filter", ct.c_float),
                 ("ms_union", ct.c_float), ("ms_sort_pass_avg", ct.c_float), ("ms_sort_hist", ct.c_float),
                 ("ms_sort_local", ct.c_float), ("sort_long_groups", ct.c_uint64), ("ms_sort_gen", ct.c_float),
-                ("parts", ct.c_int), ("sort_local_moved", ct.c_uint64), ("ms_sort_count", ct.c_float)]
+                ("parts", ct.c_int), ("sort_local_moved", ct.c_uint64), ("ms_sort_count", ct.c_float),
+                ("sort_graph", ct.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -190,11 +191,12 @@ def _buf_ptr(buf):
 class Context:
     """One device context (mp_ctx)."""
 
-    def __init__(self, k: int, min3: bool = False, device: int = 0, hbm_budget: int = 0, sort_tile: int = 0):
+    def __init__(self, k: int, min3: bool = False, device: int = 0, hbm_budget: int = 0, sort_tile: int = 0,
+                 sort_graph: bool = False):
         self.k = k
         self.min3 = bool(min3)
         cfg = Cfg(device=device, k=k, min3=1 if min3 else 0, hbm_budget=hbm_budget, sort_digit_bits=0,
-                  sort_tile=sort_tile)
+                  sort_tile=sort_tile, sort_graph=1 if sort_graph else 0)
         h = ct.c_void_p()
         rc = lib().mp_ctx_create(ct.byref(cfg), ct.byref(h))
         if rc:

@@ -23,6 +23,7 @@ int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
     if (cfg->sort_tile != 0 && cfg->sort_tile != 3072 && cfg->sort_tile != 4096 && cfg->sort_tile != 6144 &&
         cfg->sort_tile != 8192)
         return MP_EINVAL;
+    if (cfg->sort_graph != 0 && cfg->sort_graph != 1) return MP_EINVAL;
     mp_ctx *ctx = new (std::nothrow) mp_ctx();
     if (!ctx) return MP_ENOMEM;
     ctx->cfg = *cfg;
@@ -71,6 +72,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->cev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &g : ctx->sort_graphs) (void)hipGraphExecDestroy(g.second);
     for (int b = 0; b < 2; b++) {
         if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
         if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
@@ -401,6 +403,7 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
     st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen ? ctx->sort_gen_ms : 0.f;
     st.ms_sort_count = ctx->sort_count_ms;
+    st.sort_graph = ctx->sort_graph_last;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     if (stats) *stats = st;
@@ -555,30 +558,49 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     hipEventRecord(ctx->ev[0], ctx->stream);
     uint64_t n = 0;
     std::unique_ptr<GenSource> gen;
+    // every sample sorted on its own (segmented passes, each sample's tuples
+    // from its own position) unless the passes take look-back offsets
+    const char *lbe = getenv("MP_SORT_LOOKBACK");
+    const bool segmented = !ctx->cfg.min3 && !(lbe && atoi(lbe) == 1);
     if (ctx->cfg.min3) {
         MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n));
     } else {
         gen.reset(new GenSource());
+        if (segmented) MP_TRY(batch_counts(ctx));
         MP_TRY(gen_histograms(ctx, 0, 0, MP_NBUCKETS, scan_pass_mask(ctx->cfg.k), true, gen.get(), &n));
     }
     hipEventRecord(ctx->ev[1], ctx->stream);
-    // sort by (sample, k-mer, adj): the sample of a label is its chunk
-    uint64_t *d_first = nullptr;
-    MP_HIP(hipMallocAsync((void **)&d_first, sizeof(uint64_t) * (ns + 1), ctx->stream));
-    MP_HIP(hipMemcpyAsync(d_first, ctx->chunk_ids.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice,
-                          ctx->stream));
     std::vector<uint64_t> tpos(ns + 1, 0);
     int passes = 0;
-    int rc = radix_sort_tuples(ctx, n, &passes, gen.get());
-    const float gen_ms = gen ? ctx->sort_gen_ms : 0.f;
-    if (!rc) {  // then the sample of the label as the most significant digit
-        int sp = 0;
-        rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 1u << 9, &sp, d_first, ns,
-                                tpos.data());
-        passes += sp;
+    float gen_ms = 0.f;
+    if (segmented && gen->gcnt && gen->gcnt_seg && gen->seg_tuples.size() == (size_t)ns) {
+        for (int i = 0; i < ns; i++) {  // a plain run's range plan is degenerate without k-mers (mp_plan_ranges)
+            if (gen->seg_tuples[i] == 0) return ctx->fail(MP_EPLAN, "sample %d has no k-mers to partition", i);
+            tpos[i + 1] = tpos[i] + gen->seg_tuples[i];
+        }
+        SortSegments sg;
+        sg.ns = ns;
+        sg.tpos = tpos.data();
+        sg.wpos = ctx->seg_wpos.data();
+        MP_TRY(radix_sort_tuples(ctx, n, &passes, gen.get(), &sg));
+        gen_ms = ctx->sort_gen_ms;
+    } else {
+        // sort by (sample, k-mer, adj): the sample of a label is its chunk
+        uint64_t *d_first = nullptr;
+        MP_HIP(hipMallocAsync((void **)&d_first, sizeof(uint64_t) * (ns + 1), ctx->stream));
+        MP_HIP(hipMemcpyAsync(d_first, ctx->chunk_ids.data(), sizeof(uint64_t) * (ns + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+        int rc = radix_sort_tuples(ctx, n, &passes, gen.get());
+        gen_ms = gen ? ctx->sort_gen_ms : 0.f;
+        if (!rc) {  // then the sample of the label as the most significant digit
+            int sp = 0;
+            rc = radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, 1u << 9, &sp, d_first, ns,
+                                    tpos.data());
+            passes += sp;
+        }
+        MP_HIP(hipFreeAsync(d_first, ctx->stream));
+        MP_TRY(rc);
     }
-    MP_HIP(hipFreeAsync(d_first, ctx->stream));
-    MP_TRY(rc);
     if (n <= 1) {  // the sort returned early: one tuple or none, all in the sample that owns it
         uint32_t l0 = 0;
         if (n == 1) {
@@ -607,6 +629,7 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     st.sort_local_moved = ctx->local_moved;
     st.ms_sort_gen = gen_ms;
     st.ms_sort_count = ctx->sort_count_ms;
+    st.sort_graph = ctx->sort_graph_last;
     hipEventElapsedTime(&st.ms_enumerate, ctx->ev[0], ctx->ev[1]);
     hipEventElapsedTime(&st.ms_sort, ctx->ev[1], ctx->ev[7]);
     ctx->last_n = n;

@@ -97,7 +97,7 @@ struct mp_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8]{};
     hipEvent_t pev[24]{};  // per radix pass (timing of the dominant kernel)
-    hipEvent_t cev[12]{};  // per counted radix pass: before its digit count
+    hipEvent_t cev[mp::kSortPasses + 2]{};  // per counted radix pass: before its digit count (last: generating pass)
     float sort_pass_avg = 0, sort_hist_ms = 0;
     float sort_count_ms = 0;   // digit counts and offsets of the counted passes of the last sort
     float sort_gen_ms = 0;     // the generating radix pass of the last sort (0: none)
@@ -131,6 +131,21 @@ struct mp_ctx {
     mp::DevArray keys[2], lab[2], adj[2], zbits;
     mp::DevArray runbits;  // filter -> union: starts of runs to join, starts of all runs
     mp::DevArray dig, rts; // counted radix passes: next pass's digit per tuple; per-tile counts and offsets
+    mp::DevArray gcnt;     // m-mer histogram: digit counts per generating-pass tile (a scan over every k-mer)
+    mp::DevArray segtab;   // a batch's per-sample generating-pass tiles: first tile, first window
+    mp::DevArray sortaux;  // radix sort: digit bases, segment tables, generating-pass tile table
+    // captured radix pass chains (mp_cfg.sort_graph), the latest last, with the
+    // shape and buffers each was captured for (a batch sorts its tuples, the
+    // pair step's pairs and the writer's items: each chain keeps its graph)
+    std::vector<std::pair<std::vector<uint64_t>, hipGraphExec_t>> sort_graphs;
+    uint64_t sort_graph_captures = 0, sort_graph_replays = 0;
+    int sort_graph_last = 0;  // the last sort's chain: 0 launched pass by pass, 1 captured, 2 replayed
+    std::vector<uint64_t> seg_wpos;  // first window of every sample (C + 1)
+    int khist_copies = 32; // copies of the byte counts in khist (one per sample for a batch)
+    bool gcnt_seg = false; // gcnt over every sample's own tiles
+    bool gcnt_valid = false;
+    uint64_t gcnt_tile = 0, gcnt_tiles = 0;
+    int gcnt_byte = 0;
     mp::DevArray wg_cnt, wg_off, status, scratch, hist;
     mp::DevArray scanst;  // look-back status of the exclusive scans
     mp::DevArray hpart;   // m-mer histogram: tile table, partition bounds, work list
@@ -165,7 +180,7 @@ struct mp_ctx {
     {
         mp::DevArray *arrs[] = {&fq, &rec_off, &seq_off, &seq_len, &base_off, &pos_off, &packed, &nmask, &parent, &bad,
                                 &filtered, &sizes, &gparent, &gsize, &pairc, &scanst, &hpart, &khist, &labsnap,
-                                &runbits, &dig, &rts, &keys[0], &keys[1], &lab[0], &lab[1], &adj[0], &adj[1], &zbits, &wg_cnt,
+                                &runbits, &dig, &rts, &gcnt, &segtab, &sortaux, &keys[0], &keys[1], &lab[0], &lab[1], &adj[0], &adj[1], &zbits, &wg_cnt,
                                 &wg_off, &status, &scratch, &hist, &out, &items[0], &items[1], &ilens[0], &ilens[1],
                                 &iaux[0], &iaux[1]};
         for (mp::DevArray *a : arrs) f(*a);
@@ -213,6 +228,10 @@ int fill_u32(mp_ctx *ctx, void *p, uint32_t v, uint64_t n_words);
 // stages (each .hip file)
 int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_first_id, int C);
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts);
+// A batch (chunk = sample): the byte counts of every sample and the generating
+// pass's digit counts over tiles of every sample's windows (gen_histograms
+// then gives per-sample tuple counts for a segmented sort).
+int batch_counts(mp_ctx *ctx);
 // per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
 int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
 // 3-minimizer k-mers of a scan: ordered enumeration into the tuple buffer 0
@@ -227,7 +246,19 @@ uint32_t scan_pass_mask(int K);
 // holds every k-mer, so the counts of the last m-mer histogram serve.
 int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_mask, bool all, GenSource *g,
                    uint64_t *n_out);
-int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen = nullptr);
+// Samples of a batch sorted each on its own (host arrays of ns + 1 entries):
+// sample s's tuples end up in [tpos[s], tpos[s+1]); the generating pass takes
+// its windows [wpos[s], wpos[s+1]) (and the m-mer histogram's per-tile digit
+// counts over tiles of each sample's windows).
+struct SortSegments {
+    int ns = 0;
+    const uint64_t *tpos = nullptr;
+    const uint64_t *wpos = nullptr;
+};
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen = nullptr,
+                      const SortSegments *segs = nullptr);
+// tuples (k-mer windows for the generating pass) per radix tile of the ctx's tile shape
+uint64_t sort_tile_items(const mp_ctx *ctx);
 // Stable LSD passes over (keys, lab, adj) ping-pong buffers; pass p of pass_mask:
 // 0 = adj byte, 1..8 = key byte p-1, 9 = sample of the label (d_grp_first:
 // n_grp + 1 first read ids on the device; grp_base, if given, receives the
@@ -236,7 +267,7 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
 // histogram read) into the buffer *cur does not select.
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base, const GenSource *gen = nullptr);
+                       uint64_t *grp_base, const GenSource *gen = nullptr, const SortSegments *segs = nullptr);
 int iota_u32(mp_ctx *ctx, uint32_t *p, uint64_t n);
 int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st);
 // Same over segments at the given tuple positions (nseg + 1, host) with

@@ -128,7 +128,7 @@ struct RollKmer {
     }
 };
 
-// Calls fn(canonical k-mer) for every window without 'N' of a group of nr
+// Calls fn(canonical k-mer, q) for every window q without 'N' of a group of nr
 // reads (window prefix pref[0..nr], first bases base[0..nr)): each of the
 // nlanes lanes takes a contiguous run of the group's windows and rolls the
 // k-mer from one window to the next, building a read's first window whole.
@@ -150,7 +150,7 @@ __device__ __forceinline__ void for_group_windows(int K, const uint32_t *pref, c
     RollKmer k;
     k.init(K, base[lo] + p, nword, pword);
     for (;;) {
-        if (k.valid(K)) fn(k.canon(K, p));
+        if (k.valid(K)) fn(k.canon(K, p), q);
         if (++q >= qe) break;
         if (++p < pend) {
             k.roll(K, base[lo] + p + K - 1, nword, pword);
@@ -181,6 +181,13 @@ struct GenSource {
     KParams P;                                // reads, K, bucket range, labels
     uint64_t h[kSortPasses * kSortRadix];     // digit counts of the tuples
     uint64_t npos;                            // k-mer windows of all reads
+    // when set: digit counts of radix pass gcnt_pass per generating-pass tile
+    // ([tile][digit], from the m-mer histogram), so that pass needs no look-back
+    const uint32_t *gcnt = nullptr;
+    uint64_t gcnt_tiles = 0;
+    int gcnt_pass = -1;
+    bool gcnt_seg = false;  // tiles of every chunk's windows on their own (a batch's samples)
+    std::vector<uint64_t> seg_tuples;  // with gcnt_seg: the k-mers of every chunk
 };
 
 }  // namespace mp

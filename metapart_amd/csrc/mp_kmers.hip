@@ -68,7 +68,20 @@ struct HParams {
     uint32_t *tcnt;             // [c][d][tile of c] digit counts
     const uint64_t *toff;       // exclusive scan of tcnt
     uint16_t *items;            // low bucket bits, grouped by (chunk, digit)
+    // digit counts per tile of the generating radix pass (tiles of gtile
+    // consecutive windows), for a scan over every k-mer: gcnt[tile][digit] of
+    // the k-mer byte at gshift
+    uint32_t *gcnt;
+    uint64_t gtile;
+    int gshift;
+    // a batch: byte counts per chunk (bh holds C copies), and generating-pass
+    // tiles over every chunk's own windows: chunk c's tiles start at gft[c],
+    // its windows at gw[c] (C + 1 entries each)
+    int per_chunk;
+    const uint64_t *gft, *gw;
 };
+
+constexpr int kGenCntTiles = 24;  // generating-pass tiles a histogram tile counts in LDS (more: global atomics)
 
 struct HPiece {
     uint64_t b, e;   // item range
@@ -85,9 +98,13 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint64_t s_out[kHDigits];
     __shared__ uint32_t s_bh[PHASE == 0 ? 8 : 1][kSortRadix];
     __shared__ uint64_t s_words[kHWords];  // the group's 2-bit words, then its 'N' words
+    __shared__ uint32_t s_gc[PHASE == 0 ? kGenCntTiles : 1][kSortRadix];
     const int tid = threadIdx.x;
-    if (PHASE == 0)
+    if (PHASE == 0) {
         for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&s_bh[0][0])[j] = 0;
+        if (H.gcnt)
+            for (int j = tid; j < kGenCntTiles * kSortRadix; j += kKBlock) (&s_gc[0][0])[j] = 0;
+    }
     const uint32_t t = blockIdx.x;
     int c = 0, hi = H.C - 1;  // chunk: last c with tile0[c] <= t
     while (c < hi) {
@@ -98,6 +115,12 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     const uint64_t rb = H.chunk_ids[c] + (uint64_t)tl * kHReads;
     const uint64_t re = min(rb + kHReads, H.chunk_ids[c + 1]);
     const uint64_t mbase = (uint64_t)kHDigits * H.tile0[c] + tl;
+    // the generating-pass tiles this tile's windows fall in: [g_lo, g_hi];
+    // tiles count from `origin` (window) as tile `tbase`, up to window `cend`
+    const uint64_t w_lo = P.pos_off[rb], w_hi = P.pos_off[re];
+    const uint64_t origin = H.per_chunk ? H.gw[c] : 0, tbase = H.per_chunk ? H.gft[c] : 0;
+    const uint64_t cend = H.per_chunk ? H.gw[c + 1] : ~0ull;
+    const uint64_t g_lo = H.gcnt ? tbase + (w_lo - origin) / H.gtile : 0;
     if (tid < kHDigits) {
         s_dig[tid] = 0;
         if (PHASE == 1) s_out[tid] = H.toff[mbase + (uint64_t)tid * Tc];
@@ -120,7 +143,24 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
-        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon) {
+        // a lane's windows are consecutive: its generating-pass tile advances at most once per gtile windows
+        uint64_t gt = ~0ull, gnext = 0;
+        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon, uint32_t q) {
+            if (PHASE == 0 && H.gcnt) {
+                const uint64_t wi = p0 + q;
+                if (gt == ~0ull) {
+                    const uint64_t lt = (wi - origin) / H.gtile;
+                    gt = tbase + lt;
+                    gnext = origin + (lt + 1) * H.gtile;
+                }
+                while (wi >= gnext) {
+                    gt++;
+                    gnext += H.gtile;
+                }
+                const uint32_t d = (uint32_t)(canon >> H.gshift) & 0xFF;
+                if (gt - g_lo < (uint64_t)kGenCntTiles) atomicAdd(&s_gc[gt - g_lo][d], 1u);
+                else atomicAdd(&H.gcnt[gt * kSortRadix + d], 1u);
+            }
             const uint32_t b = (uint32_t)(canon >> P.bshift);
             const uint32_t d = b >> kHLowBits;
             const uint32_t slot = atomicAdd(&s_dig[d], 1u);
@@ -135,10 +175,24 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     if (PHASE == 0) {
         __syncthreads();
         if (tid < kHDigits) H.tcnt[mbase + (uint64_t)tid * Tc] = s_dig[tid];
-        unsigned long long *bh = H.bh + (uint64_t)(t % kBhRep) * 8 * kSortRadix;
+        unsigned long long *bh = H.bh + (uint64_t)(H.per_chunk ? c : t % kBhRep) * 8 * kSortRadix;
         for (int j = tid; j < 8 * kSortRadix; j += kKBlock) {
             const uint32_t v = (&s_bh[0][0])[j];
             if (v) atomicAdd(&bh[j], (unsigned long long)v);
+        }
+        if (H.gcnt && w_hi > w_lo) {
+            // a tile whose windows all lie in this tile's is written; one shared
+            // with a neighbour (the first and last) is added to
+            const uint64_t g_hi = tbase + (w_hi - 1 - origin) / H.gtile;
+            const uint64_t ng = min(g_hi - g_lo + 1, (uint64_t)kGenCntTiles);
+            for (uint64_t j = tid; j < ng * kSortRadix; j += kKBlock) {
+                const uint64_t g = g_lo + j / kSortRadix;
+                const uint32_t v = s_gc[j / kSortRadix][j % kSortRadix];
+                const uint64_t gw0 = origin + (g - tbase) * H.gtile, gw1 = min(gw0 + H.gtile, cend);
+                const bool owned = gw0 >= w_lo && gw1 <= w_hi;
+                if (owned) H.gcnt[g * kSortRadix + (j % kSortRadix)] = v;
+                else if (v) atomicAdd(&H.gcnt[g * kSortRadix + (j % kSortRadix)], v);
+            }
         }
     }
 }
@@ -187,7 +241,18 @@ __global__ __launch_bounds__(1024) void hist_count_kernel(const uint16_t *__rest
     }
 }
 
-static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk_first_id, int C, uint32_t *d_counts)
+// seg: a batch (chunk = sample): only the first phase runs, for the byte
+// counts per sample (khist, C copies) and the generating pass's digit counts
+// over tiles of every sample's own windows (d_counts unused)
+__global__ void gather_u64_kernel(const uint64_t *__restrict__ src, const uint64_t *__restrict__ idx, uint64_t n,
+                                  uint64_t *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk_first_id, int C, uint32_t *d_counts,
+                                   bool seg = false)
 {
     std::vector<uint32_t> tile0(C + 1);
     tile0[0] = 0;
@@ -206,10 +271,60 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_TRY(ensure(ctx, ctx->wg_off, (M + 1) * sizeof(uint64_t)));
     uint8_t *hb = (uint8_t *)ctx->hpart.p;
     HParams H{};
-    MP_TRY(ensure(ctx, ctx->khist, sizeof(uint64_t) * kBhRep * 8 * kSortRadix));
-    MP_TRY(fill_u32(ctx, ctx->khist.p, 0, 2 * kBhRep * 8 * kSortRadix));
+    const uint64_t ncopies = seg ? (uint64_t)C : (uint64_t)kBhRep;
+    MP_TRY(ensure(ctx, ctx->khist, sizeof(uint64_t) * ncopies * 8 * kSortRadix));
+    MP_TRY(fill_u32(ctx, ctx->khist.p, 0, 2 * ncopies * 8 * kSortRadix));
     H.bh = (unsigned long long *)ctx->khist.p;
     H.bmask = scan_pass_mask(ctx->cfg.k) & 0x1FEu;
+    H.per_chunk = seg ? 1 : 0;
+    ctx->khist_copies = (int)ncopies;
+    // digit counts per generating-pass tile of the lowest sorted byte, for a
+    // batch sorted sample by sample (its generating pass then takes counted
+    // offsets).  A plain run does not count them: at 2.4e9 windows the
+    // counted generating pass measured equal to the look-back one (27.9 ms),
+    // and counting them cost the histogram about 4 ms (LDS for 24 tiles).
+    ctx->gcnt_valid = false;
+    ctx->gcnt_seg = seg;
+    if (H.bmask && !ctx->arena && seg) {
+        const uint64_t gtile = sort_tile_items(ctx);
+        uint64_t npos = 0;
+        MP_HIP(hipMemcpyAsync(&npos, (const uint64_t *)ctx->pos_off.p + ctx->N, sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        uint64_t gtiles = (npos + gtile - 1) / gtile;
+        if (seg) {  // every sample's windows in tiles of their own
+            ctx->seg_wpos.assign(C + 1, 0);
+            MP_TRY(ensure(ctx, ctx->segtab, 3 * (uint64_t)(C + 1) * sizeof(uint64_t)));
+            uint64_t *d_w = (uint64_t *)ctx->segtab.p, *d_ids = d_w + 2 * (C + 1);
+            MP_HIP(hipMemcpyAsync(d_ids, chunk_first_id, (C + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            hipLaunchKernelGGL(gather_u64_kernel, dim3((C + 256) / 256), dim3(256), 0, ctx->stream,
+                               (const uint64_t *)ctx->pos_off.p, (const uint64_t *)d_ids, (uint64_t)(C + 1),
+                               d_w + (C + 1));
+            MP_HIP(hipGetLastError());
+            MP_HIP(hipMemcpyAsync(ctx->seg_wpos.data(), d_w + (C + 1), (C + 1) * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<uint64_t> gft(C + 1, 0);
+            for (int c = 0; c < C; c++)
+                gft[c + 1] = gft[c] + (ctx->seg_wpos[c + 1] - ctx->seg_wpos[c] + gtile - 1) / gtile;
+            gtiles = gft[C];
+            MP_HIP(hipMemcpyAsync(d_w, gft.data(), (C + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+            H.gft = d_w;
+            H.gw = d_w + (C + 1);
+            MP_HIP(hipStreamSynchronize(ctx->stream));  // gft is read by the copy above
+        }
+        if (gtiles) {
+            MP_TRY(ensure(ctx, ctx->gcnt, gtiles * kSortRadix * sizeof(uint32_t)));
+            MP_TRY(fill_u32(ctx, ctx->gcnt.p, 0, gtiles * kSortRadix));
+            H.gcnt = (uint32_t *)ctx->gcnt.p;
+            H.gtile = gtile;
+            H.gshift = 8 * (__builtin_ctz(H.bmask) - 1);
+            ctx->gcnt_tile = gtile;
+            ctx->gcnt_tiles = gtiles;
+            ctx->gcnt_byte = __builtin_ctz(H.bmask) - 1;
+        }
+    }
     H.chunk_ids = (const uint64_t *)(hb + o_ids);
     H.tile0 = (const uint32_t *)(hb + o_t0);
     H.C = C;
@@ -230,6 +345,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     const uint64_t total = bounds[nparts];
     ctx->khist_mask = H.bmask;
     ctx->khist_valid = true;
+    ctx->gcnt_valid = H.gcnt != nullptr;
+    if (seg) return 0;  // a batch's counts: nothing more
     ctx->kmer_total = total;
     if (total == 0) return 0;
     // the low bits stage in the second tuple buffer, unused until the first scan's sort
@@ -463,6 +580,13 @@ static KParams base_params(mp_ctx *ctx)
     return P;
 }
 
+int batch_counts(mp_ctx *ctx)
+{
+    KParams P = base_params(ctx);
+    P.C = ctx->C;
+    return count_plain_partitioned(ctx, P, ctx->chunk_ids.data(), ctx->C, nullptr, true);
+}
+
 int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint32_t *d_counts)
 {
     KParams P = base_params(ctx);
@@ -516,7 +640,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
         auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
-        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon) {
+        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon, uint32_t) {
             const uint64_t bk = canon >> P.bshift;
             if (bk < P.lo || bk >= P.hi) return;
 #pragma unroll
@@ -550,7 +674,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_bucket_count_kernel(KParams P, u
         for_group_windows(
             P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) { return P.nmask[w]; },
             [&](uint64_t w) { return P.packed[w]; },
-            [&](uint64_t canon) {
+            [&](uint64_t canon, uint32_t) {
                 const uint64_t bk = canon >> P.bshift;
                 if (bk >= P.lo && bk < P.hi) atomicAdd(&cnt[bk - P.lo], 1ull);
             });
@@ -628,12 +752,26 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
                           hipMemcpyDeviceToHost, ctx->stream));
     if (all && ctx->khist_valid && !(pass_mask & ~ctx->khist_mask)) {
         // every k-mer is in the scan: the m-mer histogram counted these digits already
-        std::vector<uint64_t> rep((size_t)kBhRep * 8 * kSortRadix);
+        const int nc = ctx->khist_copies;
+        std::vector<uint64_t> rep((size_t)nc * 8 * kSortRadix);
         MP_HIP(hipMemcpyAsync(rep.data(), ctx->khist.p, rep.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
         MP_HIP(hipStreamSynchronize(ctx->stream));
         memset(g->h, 0, sizeof(g->h));
-        for (int c = 0; c < kBhRep; c++)
+        for (int c = 0; c < nc; c++)
             for (int j = 0; j < 8 * kSortRadix; j++) g->h[kSortRadix + j] += rep[(size_t)c * 8 * kSortRadix + j];
+        if (ctx->gcnt_seg) {  // per sample (chunk): tuples of each, from its byte counts
+            const int b0 = __builtin_ctz(pass_mask);
+            g->seg_tuples.assign(nc, 0);
+            for (int c = 0; c < nc; c++)
+                for (int r = 0; r < kSortRadix; r++)
+                    g->seg_tuples[c] += rep[(size_t)c * 8 * kSortRadix + (size_t)(b0 - 1) * kSortRadix + r];
+        }
+        if (ctx->gcnt_valid && ctx->gcnt_tile == sort_tile_items(ctx)) {  // and per generating-pass tile
+            g->gcnt = (const uint32_t *)ctx->gcnt.p;
+            g->gcnt_tiles = ctx->gcnt_tiles;
+            g->gcnt_pass = 1 + ctx->gcnt_byte;
+            g->gcnt_seg = ctx->gcnt_seg;
+        }
     } else {
         MP_TRY(ensure(ctx, ctx->hist, sizeof(uint64_t) * (kSortPasses * kSortRadix + 64)));
         unsigned long long *gh = (unsigned long long *)ctx->hist.p;

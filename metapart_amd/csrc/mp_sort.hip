@@ -38,6 +38,7 @@
 // label (labels are read ids, samples are contiguous id ranges): the result is
 // every sample's tuples in (k-mer, adj) order, samples one after another, the
 // same as sorting each sample alone.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -78,6 +79,40 @@ template <> struct Status<true> {
     __device__ static uint64_t val(T s) { return s & ((1u << 26) - 1); }
 };
 
+// Samples of a batch sorted each on its own ("segmented"): every tile holds
+// items of one sample; sample s's tiles are [t0[s], t0[s+1]), its items
+// [pos[s], pos[s+1]) (tuples; windows for the generating pass).
+struct SegTiles {
+    const uint64_t *t0;
+    const uint64_t *pos;
+    int ns;  // 0: one segment, tiles of kTile items from 0
+};
+
+// sample of tile t: last s with t0[s] <= t
+__device__ __forceinline__ int seg_of_tile(const SegTiles &S, uint64_t t)
+{
+    int lo = 0, hi = S.ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.t0[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// items [a, e) of tile t (tiles of `tile` items, per sample when segmented)
+__device__ __forceinline__ void tile_range(const SegTiles &S, uint64_t t, uint64_t tile, uint64_t n, uint64_t &a,
+                                           uint64_t &e)
+{
+    if (S.ns == 0) {
+        a = t * tile;
+        e = min(a + tile, n);
+        return;
+    }
+    const int s = seg_of_tile(S, t);
+    a = S.pos[s] + (t - S.t0[s]) * tile;
+    e = min(a + tile, S.pos[s + 1]);
+}
+
 struct SortPass {
     const uint64_t *keys_in;
     const uint32_t *lab_in;
@@ -96,7 +131,21 @@ struct SortPass {
     const uint64_t *offs;  // counted passes: output index of each (tile, digit)'s first tuple, tile-major
     uint8_t *dig_out;      // if set: the next pass's digit of every tuple, at its output index
     int next_shift;
+    int xcd_tiles;         // counted passes: consecutive tiles on one XCD (workgroups go round-robin to 8 XCDs)
+    SegTiles seg;             // the input's tiles (scatter passes): per sample when segmented
+    const uint32_t *chk_cnt;  // counted generating pass: the per-tile counts its offsets came from, checked
+    uint32_t *chk_err;        // set when a tile's items disagree with them (the tile then writes nothing)
 };
+
+// Tile of workgroup b of a grid of g when the 8 XCDs (which take workgroups
+// round-robin) each take one contiguous range of tiles, in order: the partial
+// cache lines two neighbouring tiles write at each digit's run boundary then
+// meet in one L2.
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t g)
+{
+    const uint64_t x = b & 7, per = g >> 3, rem = g & 7;
+    return x * per + (x < rem ? x : rem) + (b >> 3);
+}
 
 // sample of a read id: last g with first[g] <= x
 __device__ __forceinline__ uint32_t group_of(const uint64_t *first, int n, uint32_t x)
@@ -156,8 +205,10 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
 // overlap its windows, the first window of read ra, the reads' bases
 // [b0, b1) and their FASTQ sequence span [f0, f1).
 struct TileInfo {
-    uint64_t ra, rb, pa, b0, b1, f0, f1, pad;
+    uint64_t ra, rb, pa, b0, b1, f0, w0;  // w0: the tile's first window
+    uint32_t flen, nw;                    // FASTQ span f0 + flen; windows of the tile
 };
+static_assert(sizeof(TileInfo) == 64, "one line");
 
 struct GenPass {
     KParams G;
@@ -165,6 +216,7 @@ struct GenPass {
     uint64_t tiles;
     uint64_t npos;         // windows of all reads
 };
+
 constexpr int kGenReads = 256;  // reads of a tile staged in LDS (more: everything read from memory)
 
 // read holding window q: last r with pos_off[r] <= q
@@ -178,19 +230,23 @@ __device__ __forceinline__ uint64_t read_of_window(const uint64_t *pos_off, uint
     return lo;
 }
 
-__global__ void tile_info_kernel(KParams G, uint64_t tile, uint64_t tiles, TileInfo *__restrict__ info)
+__global__ void tile_info_kernel(KParams G, uint64_t tile, uint64_t tiles, uint64_t npos, SegTiles S,
+                                 TileInfo *__restrict__ info)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tiles) return;
+    uint64_t w0, w1;
+    tile_range(S, t, tile, npos, w0, w1);
     TileInfo ti;
-    ti.ra = read_of_window(G.pos_off, G.N, t * tile);
-    ti.rb = t + 1 < tiles ? read_of_window(G.pos_off, G.N, (t + 1) * tile) : G.N - 1;
+    ti.ra = read_of_window(G.pos_off, G.N, w0);
+    ti.rb = read_of_window(G.pos_off, G.N, w1 - 1);
     ti.pa = G.pos_off[ti.ra];
     ti.b0 = G.base_off[ti.ra];
     ti.b1 = G.base_off[ti.rb + 1];
     ti.f0 = G.seq_off[ti.ra];
-    ti.f1 = G.seq_off[ti.rb] + G.seq_len[ti.rb];
-    ti.pad = 0;
+    ti.flen = (uint32_t)(G.seq_off[ti.rb] + G.seq_len[ti.rb] - ti.f0);
+    ti.w0 = w0;
+    ti.nw = (uint32_t)(w1 - w0);
     info[t] = ti;
 }
 
@@ -234,10 +290,12 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         grp_first = s_gbase;
     }
     if (LB != kLbCounted && tid == 0) s_tile = atomicAdd(P.tile_ctr, 1u);
-    if (LB == kLbCounted && tid == 0) s_tile = blockIdx.x;  // tiles do not wait on each other: any order
+    if (LB == kLbCounted && tid == 0)  // tiles do not wait on each other: any order
+        s_tile = P.xcd_tiles ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
     __syncthreads();
     const uint64_t tile = s_tile;
-    const uint64_t base = tile * kTile;
+    uint64_t base = tile * kTile, lim = P.n;  // the tile's items [base, lim) of the input
+    if (!GEN) tile_range(P.seg, tile, kTile, P.n, base, lim);
 
     uint64_t k[kSI];
     uint32_t l[kSI];
@@ -254,7 +312,9 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         const uint32_t nr = (uint32_t)(rb - ra + 1);
         const uint64_t pa = ti.pa;
         const uint64_t b0 = ti.b0, b1 = ti.b1;  // bases of the reads
-        const uint64_t f0 = ti.f0, f1 = ti.f1;  // their FASTQ span
+        const uint64_t f0 = ti.f0, f1 = ti.f0 + ti.flen;  // their FASTQ span
+        base = ti.w0;  // the tile's first window; its windows [base, base + nw)
+        lim = ti.w0 + ti.nw;
         const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
         const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
         const uint64_t fw0 = f0 >> 2, nf = (f1 + 3) / 4 - fw0;
@@ -305,7 +365,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             dr[i] = (uint32_t)kRadix << 16;  // not an item
             k[i] = 0;
             l[i] = 0;
-            if (idx >= A.npos) continue;
+            if (idx >= lim) continue;
             const uint32_t q = (uint32_t)(idx - pa);
             // the read holding window idx: last j with pos[j] <= q
             if (staged) {
@@ -343,7 +403,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
             const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
-            if (idx < P.n) {
+            if (idx < lim) {
                 k[i] = P.keys_in[idx];
                 l[i] = P.lab_in[idx];
                 const uint8_t a = P.adj_in[idx];
@@ -430,7 +490,16 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
     }
     if (LB == kLbCounted && tid < kRadix) {
         s_gbase[tid] = P.offs[tile * kRadix + tid] - ds;  // output index of tile position 0 of this digit
-    } else if (tid < kRadix) {
+    }
+    if (GEN && LB == kLbCounted) {
+        // offsets counted from the histogram's windows: a tile whose items differ writes nothing
+        const bool bad = tid < kRadix && P.chk_cnt[tile * kRadix + tid] != total;
+        if (__syncthreads_or(bad)) {
+            if (tid == 0) atomicOr(P.chk_err, 1u);
+            return;
+        }
+    }
+    if (LB != kLbCounted && tid < kRadix) {
         // decoupled look-back for digit tid
         uint64_t prefix = 0;
         if (tile != 0) {
@@ -518,7 +587,6 @@ static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, int lb, 
                          (unsigned long long)tiles);
     if (lb == kLb32) return launch_shape<GEN, kLb32>(ctx, SB, SI, tiles, P, A);
     if (lb == kLb64) return launch_shape<GEN, kLb64>(ctx, SB, SI, tiles, P, A);
-    if constexpr (GEN) return ctx->fail(MP_EINVAL, "the generating pass finds its offsets by look-back");
     return launch_shape<GEN, kLbCounted>(ctx, SB, SI, tiles, P, A);
 }
 
@@ -533,7 +601,7 @@ static int launch_onesweep(mp_ctx *ctx, int SB, int SI, uint64_t tiles, int lb, 
 constexpr int kCntG = 16;
 
 __global__ __launch_bounds__(256) void digit_count_kernel(const uint8_t *__restrict__ dig, uint64_t n, uint32_t tile,
-                                                          uint64_t tiles, uint32_t *__restrict__ cnt,
+                                                          uint64_t tiles, SegTiles S, uint32_t *__restrict__ cnt,
                                                           uint32_t *__restrict__ csum)
 {
     __shared__ uint32_t h[kRadix];
@@ -542,17 +610,40 @@ __global__ __launch_bounds__(256) void digit_count_kernel(const uint8_t *__restr
     for (uint64_t t = t0; t < t0 + kCntG && t < tiles; t++) {
         h[threadIdx.x] = 0;
         __syncthreads();
-        const uint64_t base = t * tile;  // tile is a multiple of 8: whole 8-byte words
-        for (uint32_t o = threadIdx.x * 8; o < tile; o += 256 * 8) {
-            const uint64_t i = base + o;
-            if (i + 8 <= n) {
-                const uint64_t v = *(const uint64_t *)(dig + i);
+        uint64_t a, e;
+        tile_range(S, t, tile, n, a, e);
+        // whole 8-byte words in the middle, single bytes at the ends
+        const uint64_t am = min(e, (a + 7) & ~7ull), em = max(am, e & ~7ull);
+        if (threadIdx.x < am - a) atomicAdd(&h[dig[a + threadIdx.x]], 1u);
+        if (threadIdx.x < e - em) atomicAdd(&h[dig[em + threadIdx.x]], 1u);
+        for (uint64_t i = am + threadIdx.x * 8; i < em; i += 256 * 8) {
+            const uint64_t v = *(const uint64_t *)(dig + i);
 #pragma unroll
-                for (int b = 0; b < 8; b++) atomicAdd(&h[(v >> (8 * b)) & 0xFF], 1u);
-            } else {
-                for (uint64_t x = i; x < n && x < i + 8; x++) atomicAdd(&h[dig[x]], 1u);
-            }
+            for (int b = 0; b < 8; b++) atomicAdd(&h[(v >> (8 * b)) & 0xFF], 1u);
         }
+        __syncthreads();
+        const uint32_t v = h[threadIdx.x];
+        cnt[t * kRadix + threadIdx.x] = v;
+        sum += v;
+        __syncthreads();
+    }
+    csum[c * kRadix + threadIdx.x] = sum;
+}
+
+// the same digit counts from the keys themselves (byte `shift / 8` of each)
+__global__ __launch_bounds__(256) void key_digit_count_kernel(const uint64_t *__restrict__ keys, uint64_t n, int shift,
+                                                              uint32_t tile, uint64_t tiles, SegTiles S,
+                                                              uint32_t *__restrict__ cnt, uint32_t *__restrict__ csum)
+{
+    __shared__ uint32_t h[kRadix];
+    const uint64_t c = blockIdx.x, t0 = c * kCntG;
+    uint32_t sum = 0;
+    for (uint64_t t = t0; t < t0 + kCntG && t < tiles; t++) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t a, e;
+        tile_range(S, t, tile, n, a, e);
+        for (uint64_t i = a + threadIdx.x; i < e; i += 256) atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
         __syncthreads();
         const uint32_t v = h[threadIdx.x];
         cnt[t * kRadix + threadIdx.x] = v;
@@ -565,7 +656,7 @@ __global__ __launch_bounds__(256) void digit_count_kernel(const uint8_t *__restr
 // coff[c][d] = base[d] + sum of csum[c'][d] over c' < c; one workgroup per digit
 __global__ __launch_bounds__(256) void chunk_scan_kernel(const uint32_t *__restrict__ csum, uint64_t nchunks,
                                                          const uint64_t *__restrict__ digit_base,
-                                                         uint64_t *__restrict__ coff)
+                                                         uint64_t *__restrict__ coff, uint64_t *__restrict__ tot)
 {
     __shared__ uint64_t s_w[256 / kWave];
     const uint32_t d = blockIdx.x;
@@ -581,33 +672,110 @@ __global__ __launch_bounds__(256) void chunk_scan_kernel(const uint32_t *__restr
         }
         if (lane == kWave - 1) s_w[wave] = inc;
         __syncthreads();
-        uint64_t woff = 0, tot = 0;
+        uint64_t woff = 0, tot_c = 0;
         for (int w = 0; w < 256 / kWave; w++) {
             woff += w < (int)wave ? s_w[w] : 0;
-            tot += s_w[w];
+            tot_c += s_w[w];
         }
         if (c < nchunks) coff[c * kRadix + d] = run + woff + inc - v;
-        run += tot;
+        run += tot_c;
         __syncthreads();
     }
+    if (tot && threadIdx.x == 0) tot[d] = run - digit_base[d];  // tuples of digit d
 }
 
-// offs[t][d] = coff[chunk of t][d] + counts of d in the chunk's tiles before t
+// csum[c][d] = counts of digit d in the kCntG tiles of chunk c
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const uint32_t *__restrict__ cnt, uint64_t tiles,
+                                                        uint32_t *__restrict__ csum)
+{
+    const uint64_t c = blockIdx.x, t0 = c * kCntG;
+    uint32_t sum = 0;
+    for (uint64_t t = t0; t < t0 + kCntG && t < tiles; t++) sum += cnt[t * kRadix + threadIdx.x];
+    csum[c * kRadix + threadIdx.x] = sum;
+}
+
+// offs[t][d] = coff[chunk of t][d] + counts of d in the chunk's tiles before
+// t, plus corr[sample of t][d] when the samples are sorted each on its own
 __global__ __launch_bounds__(256) void tile_offsets_kernel(const uint32_t *__restrict__ cnt, uint64_t tiles,
-                                                           const uint64_t *__restrict__ coff, uint64_t *__restrict__ offs)
+                                                           const uint64_t *__restrict__ coff, SegTiles S,
+                                                           const uint64_t *__restrict__ corr,
+                                                           uint64_t *__restrict__ offs)
 {
     const uint64_t c = blockIdx.x, t0 = c * kCntG;
     uint64_t run = coff[c * kRadix + threadIdx.x];
     for (uint64_t t = t0; t < t0 + kCntG && t < tiles; t++) {
-        offs[t * kRadix + threadIdx.x] = run;
+        const uint64_t add = corr ? corr[(uint64_t)seg_of_tile(S, t) * kRadix + threadIdx.x] : 0;
+        offs[t * kRadix + threadIdx.x] = run + add;
         run += cnt[t * kRadix + threadIdx.x];
     }
 }
 
+// Per-sample correction of segmented offsets.  With coff scanned from 0, the
+// running count before tile t is P(t) (digit-major over all tiles); sample s
+// holds P(t0[s+1]) - P(t0[s]) items of digit d, and its digit-d items go to
+// pos[s] + (its items of smaller digits) + (P(t) - P(t0[s])).  One workgroup
+// per sample, one lane per digit.
+__device__ __forceinline__ uint64_t prefix_before(const uint32_t *cnt, const uint64_t *coff, const uint64_t *tot,
+                                                  uint64_t tiles, uint64_t t, int d)
+{
+    if (t >= tiles) return tot[d];
+    const uint64_t c0 = t / kCntG * kCntG;
+    uint64_t v = coff[(t / kCntG) * kRadix + d];
+    for (uint64_t u = c0; u < t; u++) v += cnt[u * kRadix + d];
+    return v;
+}
+
+__global__ __launch_bounds__(256) void seg_fix_kernel(const uint32_t *__restrict__ cnt, const uint64_t *__restrict__ coff,
+                                                      const uint64_t *__restrict__ tot, uint64_t tiles, SegTiles S,
+                                                      uint64_t *__restrict__ corr)
+{
+    __shared__ uint64_t s_w[256 / kWave];
+    const int sg = blockIdx.x, d = threadIdx.x;
+    const unsigned lane = __lane_id(), wave = threadIdx.x / kWave;
+    const uint64_t p0 = prefix_before(cnt, coff, tot, tiles, S.t0[sg], d);
+    const uint64_t p1 = prefix_before(cnt, coff, tot, tiles, S.t0[sg + 1], d);
+    const uint64_t v = p1 - p0;  // items of digit d in the sample
+    uint64_t inc = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint64_t u = __shfl_up(inc, o, kWave);
+        if (lane >= (unsigned)o) inc += u;
+    }
+    if (lane == kWave - 1) s_w[wave] = inc;
+    __syncthreads();
+    uint64_t woff = 0;
+    for (int w = 0; w < (int)wave; w++) woff += s_w[w];
+    corr[(uint64_t)sg * kRadix + d] = S.pos[sg] + woff + inc - v - p0;
+}
+
+// tile shape: lanes x items.  Default from a sweep on MI355X at 2.4e9
+// tuples: 1024 x 6 (two tiles per CU, 22.8 ms/pass) < 1024 x 8 (one tile per
+// CU, 27.8) < 512 x 8 (32.8) < 512 x 16 (34.3).
+static void sort_shape(const mp_ctx *ctx, int *SB, int *SI)
+{
+    int shape = ctx->cfg.sort_tile;
+    if (const char *e = getenv("MP_SORT_TILE")) shape = atoi(e);
+    switch (shape) {
+    case 3072: *SB = 512; *SI = 6; break;
+    case 4096: *SB = 512; *SI = 8; break;
+    case 6144: *SB = 512; *SI = 12; break;
+    case 8192: *SB = 1024; *SI = 8; break;
+    default: *SB = 1024; *SI = 6; break;  // 6144 tuples on 1024 lanes
+    }
+}
+
+uint64_t sort_tile_items(const mp_ctx *ctx)
+{
+    int SB = 1024, SI = 6;
+    sort_shape(ctx, &SB, &SI);
+    return (uint64_t)SB * SI;
+}
+
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base, const GenSource *gen)
+                       uint64_t *grp_base, const GenSource *gen, const SortSegments *segs)
 {
+    if (segs && segs->ns > 0 && (!gen || !gen->gcnt || !gen->gcnt_seg))
+        return ctx->fail(MP_EINVAL, "a segmented generated sort needs the per-sample tile counts");
     *passes_out = 0;
     if (gen ? gen->npos == 0 : n <= 1) return 0;
     // histograms of every digit
@@ -662,37 +830,59 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         for (int g = 0; g < n_grp; g++) grp_base[g] = base[9][g];
         grp_base[n_grp] = n;
     }
-    // tile shape: lanes x items.  Default from a sweep on MI355X at 2.4e9
-    // tuples: 1024 x 6 (two tiles per CU, 22.8 ms/pass) < 1024 x 8 (one
-    // tile per CU, 27.8) < 512 x 8 (32.8) < 512 x 16 (34.3).
-    int shape = ctx->cfg.sort_tile;
-    if (const char *e = getenv("MP_SORT_TILE")) shape = atoi(e);
     int SB = 1024, SI = 6;
-    switch (shape) {
-    case 3072: SB = 512; SI = 6; break;
-    case 4096: SB = 512; SI = 8; break;
-    case 6144: SB = 512; SI = 12; break;
-    case 8192: SB = 1024; SI = 8; break;
-    default: SB = 1024; SI = 6; break;  // 6144 tuples on 1024 lanes
-    }
+    sort_shape(ctx, &SB, &SI);
     const uint64_t kTile = (uint64_t)SB * SI;
-    const uint64_t tiles = (n + kTile - 1) / kTile;
-    const uint64_t gtiles = gen ? (gen->npos + kTile - 1) / kTile : 0;
+    uint64_t tiles = (n + kTile - 1) / kTile;
+    uint64_t gtiles = gen ? (gen->npos + kTile - 1) / kTile : 0;
+    // segmented: the tiles of every sample (tuples; windows of the generating pass)
+    const int ns = segs ? segs->ns : 0;
+    std::vector<uint64_t> seg_host;  // tt0[ns + 1], tpos[ns + 1], gt0[ns + 1], wpos[ns + 1]
+    SegTiles segT{}, segG{}, segGout{};
+    uint64_t *d_seg = nullptr;
+    if (ns > 0) {
+        seg_host.assign(4 * (size_t)(ns + 1), 0);
+        uint64_t *tt0 = seg_host.data(), *tp = tt0 + (ns + 1), *gt0 = tp + (ns + 1), *wp = gt0 + (ns + 1);
+        for (int i = 0; i <= ns; i++) {
+            tp[i] = segs->tpos[i];
+            wp[i] = gen ? segs->wpos[i] : 0;
+        }
+        for (int i = 0; i < ns; i++) {
+            tt0[i + 1] = tt0[i] + (tp[i + 1] - tp[i] + kTile - 1) / kTile;
+            gt0[i + 1] = gt0[i] + (wp[i + 1] - wp[i] + kTile - 1) / kTile;
+        }
+        if (tp[ns] != n) return ctx->fail(MP_EINVAL, "segments hold %llu tuples, the sort %llu",
+                                          (unsigned long long)tp[ns], (unsigned long long)n);
+        tiles = tt0[ns];
+        if (gen) gtiles = gt0[ns];
+        if (gen && gtiles != gen->gcnt_tiles)
+            return ctx->fail(MP_EINVAL, "per-sample tiles differ from the histogram's");
+    }
     const uint64_t st_bytes = (tiles > gtiles ? tiles : gtiles) * kRadix * sizeof(uint64_t);
     if (ctx->status.bytes < st_bytes) {
         MP_TRY(ensure(ctx, ctx->status, st_bytes));
         MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
         ctx->epoch = 0;
     }
-    uint64_t *d_base = nullptr;
-    TileInfo *d_tile_info = nullptr;
-    MP_HIP(hipMallocAsync((void **)&d_base, sizeof(base), ctx->stream));
+    // digit bases, segment tables and the generating pass's tile table in one
+    // buffer that stays with the ctx (a captured pass chain reads them there)
+    const uint64_t aux_seg = sizeof(base), aux_ti = (aux_seg + seg_host.size() * 8 + 63) & ~63ull;
+    MP_TRY(ensure(ctx, ctx->sortaux, aux_ti + (gen ? gtiles * sizeof(TileInfo) : 0) + 64));
+    uint64_t *d_base = (uint64_t *)ctx->sortaux.p;
+    TileInfo *d_tile_info = gen ? (TileInfo *)((uint8_t *)ctx->sortaux.p + aux_ti) : nullptr;
     MP_HIP(hipMemcpyAsync(d_base, base, sizeof(base), hipMemcpyHostToDevice, ctx->stream));
+    if (ns > 0) {
+        d_seg = (uint64_t *)((uint8_t *)ctx->sortaux.p + aux_seg);
+        MP_HIP(hipMemcpyAsync(d_seg, seg_host.data(), seg_host.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        segT = SegTiles{d_seg, d_seg + (ns + 1), ns};
+        segG = SegTiles{d_seg + 2 * (ns + 1), d_seg + 3 * (ns + 1), ns};
+        segGout = SegTiles{d_seg + 2 * (ns + 1), d_seg + (ns + 1), ns};
+    }
     GenPass A{};
     if (gen) {
-        MP_HIP(hipMallocAsync((void **)&d_tile_info, gtiles * sizeof(TileInfo), ctx->stream));
         hipLaunchKernelGGL(tile_info_kernel, dim3((unsigned)((gtiles + 255) / 256)), dim3(256), 0, ctx->stream,
-                           gen->P, kTile, gtiles, d_tile_info);
+                           gen->P, kTile, gtiles, gen->npos, segG, d_tile_info);
         MP_HIP(hipGetLastError());
         A.G = gen->P;
         A.info = d_tile_info;
@@ -700,45 +890,97 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         A.npos = gen->npos;
     }
     int executed = 0;
-    // MP_SORT_STATUS64 forces the 64-bit status words on every look-back pass
-    // (tests run the small inputs through the format the largest inputs need);
-    // MP_SORT_COUNTED=1 makes the passes after the first take their offsets
-    // from digit counts (measured slower than the look-back at 2.4e9 tuples:
-    // 29.6 against 21.2 ms per pass), MP_SORT_LOOKBACK every pass by look-back
+    // The passes after the first take their offsets from digit counts, their
+    // tiles in one contiguous range per XCD.  Measured at 2.4e9 tuples (k = 31,
+    // 8-bit digits, 6144-tuple tiles), per scatter pass: look-back 21.2 ms;
+    // counted, tiles in launch order 27.3 (+ 1.2 counting); counted, tiles by
+    // XCD 17.4 (+ 1.2); counted from the keys instead of digit bytes, tiles by
+    // XCD 16.4 (+ 4.0).  The partial cache lines at the run boundaries of two
+    // neighbouring tiles are the cost: in one L2 they merge.
+    // MP_SORT_LOOKBACK=1 takes every pass by look-back, MP_SORT_XCD=0 leaves the
+    // tiles in launch order, MP_SORT_STATUS64 forces 64-bit status words on every
+    // look-back pass (tests run the small inputs through the format the
+    // largest inputs need).
     const bool force64 = getenv("MP_SORT_STATUS64") != nullptr;
-    const char *ce = getenv("MP_SORT_COUNTED");
-    const bool counted = ce && atoi(ce) == 1 && getenv("MP_SORT_LOOKBACK") == nullptr;
+    const char *lbe = getenv("MP_SORT_LOOKBACK"), *xe = getenv("MP_SORT_XCD");
+    const bool counted = !(lbe && atoi(lbe) == 1);
+    const bool count_keys = false;  // first passes of enumerated sorts count from the keys (below)
+    const bool xcd = !(xe && atoi(xe) == 0);
     std::vector<int> plist;
     for (int p = 0; p < kPasses; p++)
         if (run[p]) plist.push_back(p);
+    // the generating pass takes counted offsets too when the m-mer histogram
+    // counted its digits per tile (a scan over every k-mer, same tile size)
+    const bool gen_counted = counted && gen && gen->gcnt && gen_p == gen->gcnt_pass && gen->gcnt_tiles == gtiles &&
+                             (ns > 0) == gen->gcnt_seg;
+    if (ns > 0 && !gen_counted) return ctx->fail(MP_EINVAL, "a segmented sort takes counted offsets only");
     // counted passes: the digit bytes (n, padded to whole words) and the count/offset tables
-    const uint64_t nchunks = (tiles + kCntG - 1) / kCntG;
+    const uint64_t ctiles = tiles > gtiles ? tiles : gtiles;
+    const uint64_t nchunks = (tiles + kCntG - 1) / kCntG, cchunks = (ctiles + kCntG - 1) / kCntG;
     uint8_t *d_dig = nullptr;
     uint32_t *d_cnt = nullptr, *d_csum = nullptr;
-    uint64_t *d_coff = nullptr, *d_offs = nullptr;
-    if (counted && plist.size() > 1) {
+    uint64_t *d_coff = nullptr, *d_offs = nullptr, *d_tot = nullptr, *d_zero = nullptr, *d_corr = nullptr;
+    if (counted && (plist.size() > 1 || gen_counted)) {
         MP_TRY(ensure(ctx, ctx->dig, (n + 15) & ~7ull));
-        const uint64_t o_csum = tiles * kRadix * 4, o_coff = (o_csum + nchunks * kRadix * 4 + 7) & ~7ull,
-                       o_offs = o_coff + nchunks * kRadix * 8;
-        MP_TRY(ensure(ctx, ctx->rts, o_offs + tiles * kRadix * 8));
+        const uint64_t o_csum = ctiles * kRadix * 4, o_coff = (o_csum + cchunks * kRadix * 4 + 7) & ~7ull,
+                       o_offs = o_coff + cchunks * kRadix * 8, o_tot = o_offs + ctiles * kRadix * 8,
+                       o_zero = o_tot + kRadix * 8, o_corr = o_zero + kRadix * 8;
+        MP_TRY(ensure(ctx, ctx->rts, o_corr + (uint64_t)ns * kRadix * 8 + 64));
         d_dig = (uint8_t *)ctx->dig.p;
         d_cnt = (uint32_t *)ctx->rts.p;
         d_csum = (uint32_t *)((uint8_t *)ctx->rts.p + o_csum);
         d_coff = (uint64_t *)((uint8_t *)ctx->rts.p + o_coff);
         d_offs = (uint64_t *)((uint8_t *)ctx->rts.p + o_offs);
+        d_tot = (uint64_t *)((uint8_t *)ctx->rts.p + o_tot);
+        d_zero = (uint64_t *)((uint8_t *)ctx->rts.p + o_zero);
+        d_corr = (uint64_t *)((uint8_t *)ctx->rts.p + o_corr);
+        if (ns > 0) MP_TRY(fill_u32(ctx, d_zero, 0, 2 * kRadix));
     }
-    bool dig_valid = false;  // d_dig holds the digits of the pass about to run
+    // the generating pass's offsets, checked against the digit totals before any tuple is written
+    bool gen_offs = false;
+    if (gen_counted && d_offs) {
+        const uint64_t gchunks = (gtiles + kCntG - 1) / kCntG;
+        hipEventRecord(ctx->cev[kPasses], ctx->stream);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)gchunks), dim3(256), 0, ctx->stream, gen->gcnt, gtiles,
+                           d_csum);
+        // segmented: counts scanned from 0, then every sample's items from its own position
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3(kRadix), dim3(256), 0, ctx->stream, (const uint32_t *)d_csum,
+                           gchunks, (const uint64_t *)(ns > 0 ? d_zero : d_base + gen_p * kRadix), d_coff, d_tot);
+        if (ns > 0)
+            hipLaunchKernelGGL(seg_fix_kernel, dim3(ns), dim3(256), 0, ctx->stream, gen->gcnt,
+                               (const uint64_t *)d_coff, (const uint64_t *)d_tot, gtiles, segGout, d_corr);
+        hipLaunchKernelGGL(tile_offsets_kernel, dim3((unsigned)gchunks), dim3(256), 0, ctx->stream, gen->gcnt, gtiles,
+                           (const uint64_t *)d_coff, segGout, (const uint64_t *)(ns > 0 ? d_corr : nullptr), d_offs);
+        MP_HIP(hipGetLastError());
+        uint64_t tot[kRadix];
+        MP_HIP(hipMemcpyAsync(tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+        MP_HIP(hipStreamSynchronize(ctx->stream));
+        gen_offs = true;
+        for (int d = 0; d < kRadix; d++)
+            if (tot[d] != h[gen_p * kRadix + d]) gen_offs = false;  // not the scan's tuples: look-back
+    }
     bool counted_pass[kPasses] = {};
+    const int cur0 = *cur;
+    // The pass chain.  launch = false only advances the host state (the
+    // buffers and passes) for a replayed graph; timing records events per pass.
+    auto chain = [&](bool launch, bool timing) -> int {
+    bool dig_valid = false;  // d_dig holds the digits of the pass about to run
+    *cur = cur0;
+    executed = 0;
     for (size_t pi = 0; pi < plist.size(); pi++) {
         const int p = plist[pi];
         const int q = pi + 1 < plist.size() ? plist[pi + 1] : -1;
-        const bool this_counted = d_dig && dig_valid && p != gen_p && p >= 1 && p <= 8;
+        // a pass with no digit bytes before it (the first of an enumerated sort) counts from the keys
+        const bool from_keys = d_dig && !dig_valid && pi == 0 && p >= 1 && p <= 8 && gen_p < 0;
+        const bool this_counted =
+            (d_dig && (dig_valid || from_keys || count_keys) && p != gen_p && p >= 1 && p <= 8) ||
+            (p == gen_p && gen_offs);
         const bool next_counted = d_dig && q >= 1 && q <= 8;
         // status words: 32-bit when every digit's count fits 26 bits; the array is
         // cleared when the format changes or the epoch wraps (4 bits for 32-bit words)
         const bool s32 = dmax[p] < (1ull << 26) && !force64;
         const int lb = this_counted ? kLbCounted : s32 ? kLb32 : kLb64;
-        if (!this_counted) {
+        if (!this_counted && launch) {
             ++ctx->epoch;
             if (ctx->epoch == 0 || ctx->status_fmt != (s32 ? 1 : 2) || (s32 && (ctx->epoch & 15) == 0)) {
                 MP_TRY(fill_u32(ctx, ctx->status.p, 0, ctx->status.bytes / 4));
@@ -765,35 +1007,135 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.offs = d_offs;
         P.dig_out = next_counted ? d_dig : nullptr;
         P.next_shift = next_counted ? 8 * (q - 1) : 0;
-        if (this_counted) {
-            hipEventRecord(ctx->cev[executed], ctx->stream);
-            hipLaunchKernelGGL(digit_count_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
-                               (const uint8_t *)d_dig, n, (uint32_t)kTile, tiles, d_cnt, d_csum);
+        P.xcd_tiles = xcd ? 1 : 0;
+        P.seg = segT;
+        P.chk_cnt = p == gen_p && gen_offs ? gen->gcnt : nullptr;
+        P.chk_err = (uint32_t *)(tile_ctr + kPasses + 1);
+        if (this_counted && p != gen_p && launch) {
+            if (timing) hipEventRecord(ctx->cev[executed], ctx->stream);
+            if (count_keys || from_keys)
+                hipLaunchKernelGGL(key_digit_count_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
+                                   (const uint64_t *)keys[src].p, n, 8 * (p - 1), (uint32_t)kTile, tiles, segT,
+                                   d_cnt, d_csum);
+            else
+                hipLaunchKernelGGL(digit_count_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
+                                   (const uint8_t *)d_dig, n, (uint32_t)kTile, tiles, segT, d_cnt, d_csum);
             MP_HIP(hipGetLastError());
             hipLaunchKernelGGL(chunk_scan_kernel, dim3(kRadix), dim3(256), 0, ctx->stream, (const uint32_t *)d_csum,
-                               nchunks, (const uint64_t *)P.digit_base, d_coff);
+                               nchunks, (const uint64_t *)(ns > 0 ? d_zero : P.digit_base), d_coff, d_tot);
             MP_HIP(hipGetLastError());
+            if (ns > 0)
+                hipLaunchKernelGGL(seg_fix_kernel, dim3(ns), dim3(256), 0, ctx->stream, (const uint32_t *)d_cnt,
+                                   (const uint64_t *)d_coff, (const uint64_t *)d_tot, tiles, segT, d_corr);
             hipLaunchKernelGGL(tile_offsets_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
-                               (const uint32_t *)d_cnt, tiles, (const uint64_t *)d_coff, d_offs);
+                               (const uint32_t *)d_cnt, tiles, (const uint64_t *)d_coff, segT,
+                               (const uint64_t *)(ns > 0 ? d_corr : nullptr), d_offs);
             MP_HIP(hipGetLastError());
         }
-        counted_pass[executed] = this_counted;
-        hipEventRecord(ctx->pev[2 + 2 * executed], ctx->stream);
-        if (p == gen_p)
-            MP_TRY(launch_onesweep<true>(ctx, SB, SI, gtiles, lb, P, A));
-        else
-            MP_TRY(launch_onesweep<false>(ctx, SB, SI, tiles, lb, P, A));
-        hipEventRecord(ctx->pev[3 + 2 * executed], ctx->stream);
+        counted_pass[executed] = this_counted && p != gen_p;
+        if (launch) {
+            if (timing) hipEventRecord(ctx->pev[2 + 2 * executed], ctx->stream);
+            if (p == gen_p)
+                MP_TRY(launch_onesweep<true>(ctx, SB, SI, gtiles, lb, P, A));
+            else
+                MP_TRY(launch_onesweep<false>(ctx, SB, SI, tiles, lb, P, A));
+            if (timing) hipEventRecord(ctx->pev[3 + 2 * executed], ctx->stream);
+        }
         dig_valid = next_counted;
         *cur = dst;
         executed++;
     }
-    MP_HIP(hipFreeAsync(d_base, ctx->stream));
-    if (d_tile_info) MP_HIP(hipFreeAsync(d_tile_info, ctx->stream));
+    return 0;
+    };
+    // A chain of counted passes only (no look-back status epoch in its
+    // arguments) may be captured in a HIP graph and replayed while the sort's
+    // shape and buffers stay the same (mp_cfg.sort_graph).
+    bool graphable = ctx->cfg.sort_graph != 0;
+    {
+        bool dv = false;
+        for (size_t pi = 0; pi < plist.size() && graphable; pi++) {
+            const int p = plist[pi];
+            const int q = pi + 1 < plist.size() ? plist[pi + 1] : -1;
+            const bool from_keys = d_dig && !dv && pi == 0 && p >= 1 && p <= 8 && gen_p < 0;
+            const bool tc = (d_dig && (dv || from_keys) && p != gen_p && p >= 1 && p <= 8) || (p == gen_p && gen_offs);
+            graphable = tc;
+            dv = d_dig && q >= 1 && q <= 8;
+        }
+    }
+    if (graphable) {
+        const uint64_t key[] = {n, tiles, gtiles, (uint64_t)SB, (uint64_t)SI, (uint64_t)ns, (uint64_t)gen_p,
+                                (uint64_t)pass_mask, (uint64_t)cur0, (uint64_t)xcd, (uint64_t)(uintptr_t)keys[0].p,
+                                (uint64_t)(uintptr_t)keys[1].p, (uint64_t)(uintptr_t)lab[0].p,
+                                (uint64_t)(uintptr_t)lab[1].p, (uint64_t)(uintptr_t)adj[0].p,
+                                (uint64_t)(uintptr_t)adj[1].p, (uint64_t)(uintptr_t)d_dig, (uint64_t)(uintptr_t)d_cnt,
+                                (uint64_t)(uintptr_t)d_offs, (uint64_t)(uintptr_t)d_base,
+                                (uint64_t)(uintptr_t)d_tile_info, (uint64_t)(uintptr_t)d_seg,
+                                (uint64_t)(uintptr_t)tile_ctr, (uint64_t)(uintptr_t)(gen ? gen->gcnt : nullptr),
+                                gen ? gen->P.lo : 0, gen ? gen->P.hi : 0, gen ? (uint64_t)gen->P.scan0 : 0,
+                                (uint64_t)(uintptr_t)(gen ? gen->P.labels : nullptr), gen ? gen->P.N : 0,
+                                (uint64_t)(uintptr_t)(gen ? gen->P.packed : nullptr),
+                                (uint64_t)(uintptr_t)(gen ? gen->P.fq : nullptr), gen ? gen->npos : 0};
+        const std::vector<uint64_t> kv(key, key + sizeof(key) / sizeof(key[0]));
+        hipEventRecord(ctx->pev[2], ctx->stream);
+        constexpr size_t kGraphs = 4;  // chains kept; the oldest goes first
+        hipGraphExec_t exec = nullptr;
+        for (size_t i = 0; i < ctx->sort_graphs.size(); i++)
+            if (ctx->sort_graphs[i].first == kv) {
+                exec = ctx->sort_graphs[i].second;
+                std::rotate(ctx->sort_graphs.begin() + i, ctx->sort_graphs.begin() + i + 1, ctx->sort_graphs.end());
+                break;
+            }
+        if (exec) {
+            MP_TRY(chain(false, false));
+            ctx->sort_graph_replays++;
+            ctx->sort_graph_last = 2;
+        } else {
+            if (ctx->sort_graphs.size() >= kGraphs) {
+                (void)hipGraphExecDestroy(ctx->sort_graphs.front().second);
+                ctx->sort_graphs.erase(ctx->sort_graphs.begin());
+            }
+            hipGraph_t g = nullptr;
+            MP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+            const int rc = chain(true, false);
+            const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (ec != hipSuccess) return ctx->fail(MP_EDEVICE, "capture of the radix passes: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return ctx->fail(MP_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+            ctx->sort_graphs.emplace_back(kv, exec);
+            ctx->sort_graph_captures++;
+            ctx->sort_graph_last = 1;
+        }
+        MP_HIP(hipGraphLaunch(exec, ctx->stream));
+        hipEventRecord(ctx->pev[3], ctx->stream);
+    } else {
+        MP_TRY(chain(true, true));
+        ctx->sort_graph_last = 0;
+    }
+
+    uint32_t chk_err = 0;
+    if (gen_offs) MP_HIP(hipMemcpyAsync(&chk_err, tile_ctr + kPasses + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     MP_HIP(hipStreamSynchronize(ctx->stream));
+    if (chk_err)
+        return ctx->fail(MP_EDEVICE, "generating pass: a tile's k-mers differ from the histogram's counts for it");
     float tot = 0, ms = 0;
     ctx->sort_gen_ms = 0;
     ctx->sort_count_ms = 0;
+    if (graphable) {  // one graph: its time spread over its passes
+        hipEventElapsedTime(&ms, ctx->pev[2], ctx->pev[3]);
+        ctx->sort_pass_avg = executed ? ms / executed : 0.f;
+        hipEventElapsedTime(&ctx->sort_hist_ms, ctx->pev[0], ctx->pev[1]);
+        *passes_out = executed;
+        return 0;
+    }
+    if (gen_counted && d_offs) {  // the generating pass's offsets (from the histogram's counts)
+        hipEventElapsedTime(&ms, ctx->cev[kPasses], ctx->pev[2]);
+        ctx->sort_count_ms += ms;
+    }
     for (int e = 0; e < executed; e++) {
         if (counted_pass[e]) {
             hipEventElapsedTime(&ms, ctx->cev[e], ctx->pev[2 + 2 * e]);
@@ -837,11 +1179,25 @@ struct LocalPass {  // the tuple arrays, sorted in place
     uint64_t *ovf;              // starts of groups longer than kLMax
     unsigned long long *novf;
     uint32_t *moved;            // per wave: tuples written elsewhere than they were (summed afterwards)
+    const uint64_t *seg;        // samples sorted each on its own: their first tuples (nseg + 1), group starts too
+    int nseg;
 };
 
-__device__ __forceinline__ bool group_start(const uint64_t *keys, uint64_t i, int shift)
+// first sample start >= i (seg[nseg] = n)
+__device__ __forceinline__ uint64_t next_seg_start(const uint64_t *seg, int nseg, uint64_t i)
 {
-    return i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift);
+    int lo = 0, hi = nseg;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (seg[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    return seg[lo];
+}
+
+__device__ __forceinline__ bool group_start(const LocalPass &L, uint64_t i)
+{
+    if (i == 0 || (L.keys[i] >> L.shift) != (L.keys[i - 1] >> L.shift)) return true;
+    return L.nseg && next_seg_start(L.seg, L.nseg, i) == i;
 }
 
 __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
@@ -870,6 +1226,11 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         if (lane == 0) s_st[j0 / kWave] = m;
     }
     __syncthreads();
+    if (L.nseg && threadIdx.x == 0) {  // the samples' first tuples start groups too
+        for (uint64_t p = next_seg_start(L.seg, L.nseg, w0); p < w0 + nld; p = next_seg_start(L.seg, L.nseg, p + 1))
+            s_st[(p - w0) / kWave] |= 1ull << ((p - w0) % kWave);
+    }
+    __syncthreads();
     const uint32_t e0 = (uint32_t)(w1 - w0), nwl = (nld + kWave - 1) / kWave;
     // first start in [w0, w1): none means no group starts here
     uint32_t r0 = e0;
@@ -893,7 +1254,7 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     if (r1 >= nld && w0 + nld < lim) {
         for (uint64_t c0 = w0 + nld; c0 < lim; c0 += kLThreads) {
             const uint64_t i = c0 + threadIdx.x;
-            const bool st = i < lim && group_start(L.keys, i, L.shift);
+            const bool st = i < lim && group_start(L, i);
             const uint64_t m = __ballot(st);
             if (lane == 0 && m) atomicMin(&s_r1, (unsigned long long)(c0 + wave * kWave + (uint64_t)__builtin_ctzll(m)));
             if (__syncthreads_or(st)) break;
@@ -1003,12 +1364,14 @@ __global__ __launch_bounds__(256) void sum_u32_kernel(const uint32_t *__restrict
 
 // end of each long group: the next group start
 __global__ void long_end_kernel(const uint64_t *__restrict__ keys, uint64_t n, int shift,
-                                const uint64_t *__restrict__ starts, uint64_t nl, uint64_t *__restrict__ ends)
+                                const uint64_t *__restrict__ starts, uint64_t nl, const uint64_t *__restrict__ seg,
+                                int nseg, uint64_t *__restrict__ ends)
 {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nl) return;
     const uint64_t s = starts[g], hk = keys[s] >> shift;
-    uint64_t lo = s + 1, hi = n;  // first index > s with a different high part (sorted by it)
+    // first index > s with a different high part (sorted by it), at most the next sample's start
+    uint64_t lo = s + 1, hi = nseg ? next_seg_start(seg, nseg, s + 1) : n;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if ((keys[mid] >> shift) == hk) lo = mid + 1; else hi = mid;
@@ -1037,14 +1400,14 @@ uint32_t scan_pass_mask(int K)
     return mask;
 }
 
-int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen)
+int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource *gen, const SortSegments *segs)
 {
     // k-mer bytes only: within a run nothing downstream depends on the adj
     // order except the quirk's first tuples and the tuple dump, which derive it
     const int S = sort_local_bits(ctx->cfg.k);
     const uint32_t mask = scan_pass_mask(ctx->cfg.k);
     MP_TRY(radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, mask, passes_out, nullptr, 0,
-                              nullptr, gen));
+                              nullptr, gen, segs));
     if (S == 0 || n <= 1) return 0;
     const uint64_t wins = (n + kLWin - 1) / kLWin;
     // scratch: long-group count, moved-tuple total, long-group starts (one per window at most),
@@ -1065,6 +1428,16 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     L.ovf = (uint64_t *)ctx->scratch.p + 2;
     L.novf = novf;
     L.moved = d_moved;
+    L.seg = nullptr;
+    L.nseg = 0;
+    uint64_t *d_lseg = nullptr;
+    if (segs && segs->ns > 0) {
+        MP_TRY(tmp_alloc(ctx, (void **)&d_lseg, (segs->ns + 1) * sizeof(uint64_t)));
+        MP_HIP(hipMemcpyAsync(d_lseg, segs->tpos, (segs->ns + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        L.seg = d_lseg;
+        L.nseg = segs->ns;
+    }
     if (!grid_fits(wins, kLThreads))
         return ctx->fail(MP_EINVAL, "local pass of %llu windows exceeds one dispatch: use more scans",
                          (unsigned long long)wins);
@@ -1089,7 +1462,8 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         uint64_t *d_ends = nullptr;
         MP_HIP(hipMallocAsync((void **)&d_ends, nl * sizeof(uint64_t), ctx->stream));
         hipLaunchKernelGGL(long_end_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const uint64_t *)L.keys, n, S, (const uint64_t *)L.ovf, nl, d_ends);
+                           (const uint64_t *)L.keys, n, S, (const uint64_t *)L.ovf, nl, (const uint64_t *)L.seg, L.nseg,
+                           d_ends);
         MP_HIP(hipGetLastError());
         MP_HIP(hipMemcpyAsync(se.data(), L.ovf, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         MP_HIP(hipMemcpyAsync(se.data() + nl, d_ends, nl * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1099,6 +1473,7 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         for (int b = 0; b < S / 8; b++) lmask |= 1u << (b + 1);
         const float keep_avg = ctx->sort_pass_avg, keep_hist = ctx->sort_hist_ms,
                     keep_gen = ctx->sort_gen_ms, keep_count = ctx->sort_count_ms;  // stats of the main passes
+        const int keep_graph = ctx->sort_graph_last;
         for (uint64_t g = 0; g < nl; g++) {
             const uint64_t a = se[g], e = se[nl + g];
             DevArray kv[2], lv[2], av[2];
@@ -1121,9 +1496,11 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         MP_HIP(hipStreamSynchronize(ctx->stream));
         ctx->sort_pass_avg = keep_avg;
         ctx->sort_count_ms = keep_count;
+        ctx->sort_graph_last = keep_graph;
         ctx->sort_hist_ms = keep_hist;
         ctx->sort_gen_ms = keep_gen;
     }
+    if (d_lseg) MP_TRY(tmp_free(ctx, d_lseg));
     *passes_out += 1;
     return 0;
 }

@@ -42,9 +42,15 @@ def _solo(buf, out_dir):
     return r
 
 
+# segmented: every sample's tuples sorted in tiles of their own, from its own
+# position (the default); sample_pass: look-back passes over all samples, then
+# a last pass on the sample of the label (MP_SORT_LOOKBACK=1)
 @pytest.mark.gpu
-def test_batch_equals_solo_runs(tmp_path):
-    print("gpu batch: 6 samples vs solo runs", flush=True)
+@pytest.mark.parametrize("mode", ["segmented", "sample_pass"])
+def test_batch_equals_solo_runs(mode, tmp_path, monkeypatch):
+    print("gpu batch: 6 samples vs solo runs, %s sort" % mode, flush=True)
+    if mode == "sample_pass":
+        monkeypatch.setenv("MP_SORT_LOOKBACK", "1")
     bufs = _samples()
     solo = []
     for i, b in enumerate(bufs):
@@ -112,3 +118,38 @@ def test_batch_rejects_bad_samples():
         res, _, _ = ctx.batch_partition()
         assert np.array_equal(res[0]["W"], res[1]["W"])
         assert np.array_equal(ctx.batch_parent(0), ctx.batch_parent(1))
+
+
+@pytest.mark.gpu
+def test_batch_with_captured_pass_chain_equals_launched(tmp_path):
+    """mp_cfg.sort_graph: the batch's radix pass chains (its tuples', and the
+    writer's items') are captured in HIP graphs on the first run and replayed
+    on the next ones (same shape and buffers); every replay, with other work
+    on the device in between, gives the labels, plans and output files of the
+    pass-by-pass launch."""
+    print("gpu batch: captured pass chains, 3 replays", flush=True)
+    bufs = _samples()
+    with mp.Context(k=K) as ctx:
+        ctx.batch_load(bufs)
+        want, st0, _ = ctx.batch_partition()
+        assert st0.sort_graph == 0
+        want_par = [ctx.batch_parent(i) for i in range(len(bufs))]
+        ctx.batch_write(str(tmp_path / "eager"))
+    other = mp.synth(0, n_reads=20000, seed=3)
+    with mp.Context(k=K, sort_graph=True) as ctx, mp.Context(k=K) as side:
+        ctx.batch_load(bufs)
+        modes = []
+        for rep in range(4):
+            res, st, _ = ctx.batch_partition()
+            modes.append(st.sort_graph)
+            for a, b in zip(res, want):
+                assert np.array_equal(a["W"], b["W"]) and a["components"] == b["components"], rep
+            for i in range(len(bufs)):
+                assert np.array_equal(ctx.batch_parent(i), want_par[i]), (rep, i)
+            ctx.batch_write(str(tmp_path / ("graph%d" % rep)))
+            for i in range(len(bufs)):
+                assert _hash_dir(str(tmp_path / ("graph%d" % rep) / ("sample%d" % i))) == \
+                    _hash_dir(str(tmp_path / "eager" / ("sample%d" % i))), (rep, i)
+            side.load(other, [0, other.nbytes])  # eager work between the replays
+            mp.partition(side, S=1, T=1, build_output=False)
+        assert modes == [1, 2, 2, 2], modes
