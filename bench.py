@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch the batch's radix passes one by one")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--serve-jobs", type=int, default=200)
+    ap.add_argument("--serve-engines", type=int, default=3)
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="arrival rate as a fraction of the plain path's job rate at the mean job")
     ap.add_argument("--no-c3", action="store_true", help="skip the large out-of-core share (100M reads)")
@@ -407,9 +408,10 @@ def served_section(args, mp):
     """Poisson-arriving partition jobs (job j: seed 5000 + j, its own pool of
     20-100 genomes of 0.5-3 Mb, log-uniform 50k-2M reads of 150 bp, 1 % errors)
     from host memory, at a rate of serve_load x the plain path's job rate at
-    the mean job (529 k reads): the request loop (2 engines, HBM-budget
+    the mean job (529 k reads): the request loop (engines, HBM-budget
     admission, continuous batching) against the plain per-job path (one
-    context, jobs one at a time in arrival order), same arrivals."""
+    context, jobs one at a time in arrival order), same arrivals; then every
+    job queued at once (saturation) against the plain path back to back."""
     import numpy as np
     params = mp.job_params(n_jobs=args.serve_jobs, seed=7)
     t0 = time.time()
@@ -449,7 +451,8 @@ def served_section(args, mp):
     ctx.close()
 
     # served
-    srv = mp.Server(k=args.k, engines=2)
+    E = args.serve_engines
+    srv = mp.Server(k=args.k, engines=E)
     srv.wait(srv.submit(warm))
     jobs = []
     start = time.perf_counter()
@@ -459,9 +462,21 @@ def served_section(args, mp):
             time.sleep(arrivals[i] - now)
         jobs.append(srv.submit(b))
     res = [srv.wait(j)[0] for j in jobs]
-    srv.close()
     lat = [r["t_done"] - r["t_submit"] for r in res]
     span = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    # saturation: every job queued at once, against the plain path's back-to-back rate
+    t0 = time.perf_counter()
+    sat = [srv.submit(b) for b in bufs]
+    sres = [srv.wait(j)[0] for j in sat]
+    sat_s = time.perf_counter() - t0
+    srv.close()
+    ctx = mp.Context(k=args.k)
+    t0 = time.perf_counter()
+    for b in bufs:
+        ctx.load(b, [0, b.nbytes])
+        mp.partition(ctx, S=1, T=1, build_output=False)
+    plain_sat_s = time.perf_counter() - t0
+    ctx.close()
     for b in bufs:
         b.free()
     warm.free()
@@ -472,10 +487,14 @@ def served_section(args, mp):
             "jobs": len(bufs), "reads": int(sum(reads)), "rate_jobs_per_s": rate,
             "plain_capacity_jobs_per_s": capacity, "mean_job_ms": 1e3 / capacity,
             "served": {"p50_ms": 1e3 * _pct(lat, 50), "p99_ms": 1e3 * _pct(lat, 99),
-                       "reads_per_s": sum(reads) / span, "engines": 2,
+                       "reads_per_s": sum(reads) / span, "engines": E,
                        "mean_batch_jobs": float(np.mean([r["batch_jobs"] for r in res]))},
             "plain": {"p50_ms": 1e3 * _pct(lat_plain, 50), "p99_ms": 1e3 * _pct(lat_plain, 99),
                       "reads_per_s": sum(reads) / plain_span},
+            "saturation": {"served_reads_per_s": sum(reads) / sat_s, "plain_reads_per_s": sum(reads) / plain_sat_s,
+                           "speedup": plain_sat_s / sat_s,
+                           "served_mean_batch_jobs": float(np.mean([r["batch_jobs"] for r in sres])),
+                           "setup": "all %d jobs queued at once (served) / run back to back (plain)" % len(bufs)},
             "generate_s": gen_s}
 
 

@@ -255,7 +255,7 @@ typedef struct mp_server mp_server;
 typedef struct {
     int device;
     int k;
-    int engines;              /* concurrent contexts (0: 2) */
+    int engines;              /* concurrent contexts (0: 3) */
     int max_jobs_per_batch;   /* 0: 64; at most MP_MAX_SAMPLES */
     uint64_t hbm_budget;      /* device bytes for all engines (0: 3/4 of free memory) */
 } mp_server_cfg;

@@ -349,7 +349,7 @@ class Context:
 class Server:
     """The served form (mp_server): submit jobs at any time, wait for each."""
 
-    def __init__(self, k: int, engines: int = 2, max_jobs_per_batch: int = 64, hbm_budget: int = 0, device: int = 0):
+    def __init__(self, k: int, engines: int = 3, max_jobs_per_batch: int = 64, hbm_budget: int = 0, device: int = 0):
         cfg = ServerCfg(device=device, k=k, engines=engines, max_jobs_per_batch=max_jobs_per_batch,
                         hbm_budget=hbm_budget)
         h = ct.c_void_p()

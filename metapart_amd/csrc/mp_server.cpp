@@ -51,7 +51,9 @@ uint64_t estimate_bytes(uint64_t nbytes) { return nbytes + nbytes + (nbytes / 2)
 // FASTQ bytes of one batch (a larger job runs alone): about 800 k reads of
 // 150 bp, a few ms of device work, so the jobs of a batch, which all finish
 // with it, do not wait on a long one; batching pays off for the small jobs,
-// whose fixed per-sample costs it shares.
+// whose fixed per-sample costs it shares.  Under a backlog, batches that grew
+// with the queue (the queued bytes shared among the engines) moved fewer reads
+// per second than this cap: a large batch leaves the other engines idle.
 constexpr uint64_t kBatchBytes = 256ull << 20;
 
 }  // namespace
@@ -156,7 +158,7 @@ int mp_server_create(const mp_server_cfg *cfg, mp_server **out)
     if (!cfg || !out || cfg->k < 11 || cfg->k > 32) return MP_EINVAL;
     auto *s = new mp_server();
     s->cfg = *cfg;
-    if (s->cfg.engines <= 0) s->cfg.engines = 2;
+    if (s->cfg.engines <= 0) s->cfg.engines = 3;
     if (s->cfg.max_jobs_per_batch <= 0 || s->cfg.max_jobs_per_batch > MP_MAX_SAMPLES)
         s->cfg.max_jobs_per_batch = 64;
     if (hipSetDevice(s->cfg.device) != hipSuccess) {

@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mp_internal.h"
@@ -346,59 +347,95 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         uint8_t *s_map = (uint8_t *)s_buf + mapo;
         const uint32_t q0 = (uint32_t)(base - pa);  // the tile's first window, relative to the first read
         if (staged) {
-            for (uint32_t j = tid; j < nr; j += kSB) {
-                const uint32_t a = max(s_rpos[j], q0), e = min(s_rpos[j + 1], q0 + (uint32_t)kTile);
-                for (uint32_t w = a; w < e; w++) s_map[w - q0] = (uint8_t)j;
+            // lane tid fills entries [tid * kSI, tid * kSI + kSI): the read of its first
+            // window by binary search (the last read whose first window is at or before
+            // it; reads without windows share the next read's), then onwards
+            const uint32_t w0 = q0 + (uint32_t)tid * kSI;
+            uint32_t lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (s_rpos[mid] <= w0) lo = mid; else hi = mid - 1;
+            }
+#pragma unroll
+            for (int i = 0; i < kSI; i++) {
+                while (lo + 1 < nr && s_rpos[lo + 1] <= w0 + i) lo++;
+                s_map[(uint32_t)tid * kSI + i] = (uint8_t)lo;
             }
             __syncthreads();
         }
-        auto rpos = [&](uint32_t j) -> uint32_t { return staged ? s_rpos[j] : (uint32_t)(A.G.pos_off[ra + j] - pa); };
-        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_nw[w - wn0] : A.G.nmask[w]; };
-        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_pw[w - wp0] : A.G.packed[w]; };
-        auto fqb = [&](uint64_t x) -> uint8_t {
-            return staged ? (uint8_t)(s_fw[(x >> 2) - fw0] >> (8 * (x & 3))) : A.G.fq[x];
-        };
-        uint32_t lo = 0;  // read of the lane's last item: its items are 64 windows apart, so a few steps on
+        // the items, with the read data in LDS (ST) or in memory: two copies of
+        // the loop, so that no access tests which
+        auto items = [&](auto st_tag) {
+            constexpr bool ST = decltype(st_tag)::value;
+            auto rpos = [&](uint32_t j) -> uint32_t {
+                if constexpr (ST) return s_rpos[j];
+                else return (uint32_t)(A.G.pos_off[ra + j] - pa);
+            };
+            auto nword = [&](uint64_t w) -> uint64_t {
+                if constexpr (ST) return s_nw[w - wn0];
+                else return A.G.nmask[w];
+            };
+            auto pword = [&](uint64_t w) -> uint64_t {
+                if constexpr (ST) return s_pw[w - wp0];
+                else return A.G.packed[w];
+            };
+            auto fqb = [&](uint64_t x) -> uint8_t {
+                if constexpr (ST) return (uint8_t)(s_fw[(x >> 2) - fw0] >> (8 * (x & 3)));
+                else return A.G.fq[x];
+            };
+            uint32_t lo = 0;  // read of the lane's last item: its items are 64 windows apart, so a few steps on
 #pragma unroll
-        for (int i = 0; i < kSI; i++) {
-            const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
-            dr[i] = (uint32_t)kRadix << 16;  // not an item
-            k[i] = 0;
-            l[i] = 0;
-            if (idx >= lim) continue;
-            const uint32_t q = (uint32_t)(idx - pa);
-            // the read holding window idx: last j with pos[j] <= q
-            if (staged) {
-                lo = s_map[q - q0];
-            } else {
-                for (int st = 0; st < 3 && lo + 1 < nr && rpos(lo + 1) <= q; st++) lo++;
-                if (lo + 1 < nr && rpos(lo + 1) <= q) {
-                    uint32_t hi = nr - 1;
-                    lo++;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi + 1) >> 1;
-                        if (rpos(mid) <= q) lo = mid; else hi = mid - 1;
+            for (int i = 0; i < kSI; i++) {
+                const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+                dr[i] = (uint32_t)kRadix << 16;  // not an item
+                k[i] = 0;
+                l[i] = 0;
+                if (idx >= lim) continue;
+                const uint32_t q = (uint32_t)(idx - pa);
+                // the read holding window idx: last j with pos[j] <= q
+                if constexpr (ST) {
+                    lo = s_map[q - q0];
+                } else {
+                    for (int st = 0; st < 3 && lo + 1 < nr && rpos(lo + 1) <= q; st++) lo++;
+                    if (lo + 1 < nr && rpos(lo + 1) <= q) {
+                        uint32_t hi = nr - 1;
+                        lo++;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi + 1) >> 1;
+                            if (rpos(mid) <= q) lo = mid; else hi = mid - 1;
+                        }
                     }
                 }
+                const uint64_t r = ra + lo;
+                const uint32_t p = q - rpos(lo);
+                uint64_t rbase, so;
+                uint32_t L;
+                if constexpr (ST) {
+                    rbase = s_rbase[lo];
+                    so = s_rso[lo];
+                    L = s_rlen[lo];
+                } else {
+                    rbase = A.G.base_off[r];
+                    so = A.G.seq_off[r];
+                    L = A.G.seq_len[r];
+                }
+                uint64_t canon;
+                bool fwd;
+                if (!kmer_from(A.G.K, rbase + p, p, nword, pword, canon, fwd)) continue;
+                const uint64_t bk = canon >> A.G.bshift;
+                if (bk < A.G.lo || bk >= A.G.hi) continue;
+                k[i] = canon;
+                const uint8_t prev = p == 0 ? (uint8_t)'N' : fqb(so + p - 1);
+                const uint8_t next = p == L - (uint32_t)A.G.K ? (uint8_t)'N' : fqb(so + p + A.G.K);
+                const uint8_t a = adj_from(prev, next, fwd);
+                ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
+                if constexpr (ST) l[i] = s_rlab[lo];
+                else l[i] = label_of(A.G, r);
+                dr[i] = digit_of(k[i], a, P.shift) << 16;
             }
-            const uint64_t r = ra + lo;
-            const uint32_t p = q - rpos(lo);
-            const uint64_t rbase = staged ? s_rbase[lo] : A.G.base_off[r];
-            uint64_t canon;
-            bool fwd;
-            if (!kmer_from(A.G.K, rbase + p, p, nword, pword, canon, fwd)) continue;
-            const uint64_t bk = canon >> A.G.bshift;
-            if (bk < A.G.lo || bk >= A.G.hi) continue;
-            k[i] = canon;
-            const uint64_t so = staged ? s_rso[lo] : A.G.seq_off[r];
-            const uint32_t L = staged ? s_rlen[lo] : A.G.seq_len[r];
-            const uint8_t prev = p == 0 ? (uint8_t)'N' : fqb(so + p - 1);
-            const uint8_t next = p == L - (uint32_t)A.G.K ? (uint8_t)'N' : fqb(so + p + A.G.K);
-            const uint8_t a = adj_from(prev, next, fwd);
-            ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
-            l[i] = staged ? s_rlab[lo] : label_of(A.G, r);
-            dr[i] = digit_of(k[i], a, P.shift) << 16;
-        }
+        };
+        if (staged) items(std::true_type{});
+        else items(std::false_type{});
     } else {
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
