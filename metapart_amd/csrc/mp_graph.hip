@@ -322,6 +322,9 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     }
     __syncthreads();
     if (threadIdx.x * 64 < cnt) rs_bits[t0 / 64 + threadIdx.x] = s_rs[threadIdx.x];
+    // the lane's filtered tuples first (bit q), then their labels loaded
+    // together, then the marks: one load latency instead of one per tuple
+    uint32_t fm = 0;
 #pragma unroll
     for (int q = 0; q < kFIter; q++) {
         const uint32_t j = q * kFLanes + threadIdx.x;
@@ -335,8 +338,19 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         const uint32_t c = s_vd[sj];
         if (!(c & kVMark)) continue;
         const uint32_t v = (c & 0x1F) | (((c >> 5) & 7) << 8) | (((c >> 8) & 7) << 12);
-        if (tuple_filtered(v, q < 8 ? (uint32_t)(ap0 >> (8 * q)) & 0xFF : ap1)) G.filtered[G.lab[t0 + j]] = 1;
+        if (tuple_filtered(v, q < 8 ? (uint32_t)(ap0 >> (8 * q)) & 0xFF : ap1)) fm |= 1u << q;
     }
+    uint32_t lv[kFIter];
+#pragma unroll
+    for (int q = 0; q < kFIter; q++)
+        if ((fm >> q) & 1) lv[q] = G.lab[t0 + q * kFLanes + threadIdx.x];
+    // every load above completes here, once: the stores below then wait on nothing
+    // (left to itself the compiler waits before each store, the previous store included)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged (gfx9 encoding)
+#pragma unroll
+    for (int q = 0; q < kFIter; q++)
+        if ((fm >> q) & 1) G.filtered[lv[q]] = 1;
     block_add(ev, &G.stats[0]);
     block_add(z, &G.stats[1]);
 }
@@ -419,11 +433,15 @@ __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *_
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
     for (uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave * kUW; w0 < nw;
          w0 += nwaves * kUW) {
+        // the words' bitmaps and every lane's labels, loaded together (a label
+        // is used only in a joined run, but the lines are fetched anyway)
         uint64_t st[kUW], rs[kUW];
+        uint32_t lb[kUW];
 #pragma unroll
         for (int u = 0; u < kUW; u++) {
             st[u] = w0 + u < nw ? st_bits[w0 + u] : 0;
             rs[u] = w0 + u < nw ? rs_bits[w0 + u] : 0;
+            lb[u] = G.lab[min((w0 + u) * 64 + lane, G.n - 1)];
         }
         // the run holding tuple w0 * 64 when it began before: its start and whether it is joined
         uint64_t cs = 0;
@@ -469,7 +487,7 @@ __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *_
                 join = (rs[v] >> (ucs[u] % 64)) & 1;
             }
             const uint64_t t = (w0 + u) * 64 + lane;
-            b[u] = t < G.n && join ? G.lab[t] : 0;
+            b[u] = t < G.n && join ? lb[u] : 0;
             live[u] = t < G.n && join;
         }
 #pragma unroll

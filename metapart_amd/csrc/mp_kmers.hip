@@ -89,7 +89,10 @@ struct HPiece {
     uint32_t split;  // partition spread over several pieces: add atomically
 };
 
-template <int PHASE>  // 0: digit counts per tile, 1: scatter of the low bits
+// PHASE 0: digit counts per tile (GC: and the generating pass's digit counts
+// per tile, H.gcnt, whose LDS copy would otherwise cost the plain form two
+// thirds of its workgroups per CU), 1: scatter of the low bits
+template <int PHASE, bool GC = false>
 __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H)
 {
     __shared__ uint32_t s_pref[kRG + 1];
@@ -98,11 +101,11 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint64_t s_out[kHDigits];
     __shared__ uint32_t s_bh[PHASE == 0 ? 8 : 1][kSortRadix];
     __shared__ uint64_t s_words[kHWords];  // the group's 2-bit words, then its 'N' words
-    __shared__ uint32_t s_gc[PHASE == 0 ? kGenCntTiles : 1][kSortRadix];
+    __shared__ uint32_t s_gc[GC ? kGenCntTiles : 1][kSortRadix];
     const int tid = threadIdx.x;
     if (PHASE == 0) {
         for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&s_bh[0][0])[j] = 0;
-        if (H.gcnt)
+        if (GC)
             for (int j = tid; j < kGenCntTiles * kSortRadix; j += kKBlock) (&s_gc[0][0])[j] = 0;
     }
     const uint32_t t = blockIdx.x;
@@ -128,17 +131,31 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     for (uint64_t r0 = rb; r0 < re; r0 += kRG) {
         const int nr = (int)min((uint64_t)kRG, re - r0);
         __syncthreads();  // previous group done with s_pref / s_base
+        // the group's read table and stream words (a group of very long reads
+        // reads memory): every lane's loads are issued before its stores, the
+        // indices clamped instead of branched on
         const uint64_t p0 = P.pos_off[r0];
-        if (tid <= nr) s_pref[tid] = (uint32_t)(P.pos_off[r0 + tid] - p0);
-        if (tid < nr) s_base[tid] = P.base_off[r0 + tid];
-        // stage the group's stream words (a group of very long reads reads memory)
+        const uint64_t v_pos = P.pos_off[r0 + min(tid, nr)], v_base = P.base_off[r0 + min(tid, nr - 1)];
         const uint64_t b0 = P.base_off[r0], b1 = P.base_off[r0 + nr];
         const uint64_t wp0 = b0 >> 5, np = b1 > b0 ? ((b1 - 1) >> 5) + 2 - wp0 : 0;
         const uint64_t wn0 = b0 >> 6, nn = b1 > b0 ? ((b1 - 1) >> 6) + 2 - wn0 : 0;
-        const bool staged = np + nn <= (uint64_t)kHWords;
+        const bool staged = nn && np + nn <= (uint64_t)kHWords;
+        constexpr int kHW = kHWords / kKBlock;
+        static_assert(kHW * kKBlock == kHWords, "staged words per lane");
+        uint64_t v_w[kHW];
         if (staged) {
-            for (uint32_t j = tid; j < np; j += kKBlock) s_words[j] = P.packed[wp0 + j];
-            for (uint32_t j = tid; j < nn; j += kKBlock) s_words[np + j] = P.nmask[wn0 + j];
+#pragma unroll
+            for (int q = 0; q < kHW; q++) {  // word j: 2-bit word j (< np), else 'N' word j - np
+                const uint64_t j = (uint64_t)q * kKBlock + tid;
+                v_w[q] = j < np ? P.packed[wp0 + j] : P.nmask[wn0 + min(j - np, nn - 1)];
+            }
+        }
+        if (tid <= nr) s_pref[tid] = (uint32_t)(v_pos - p0);
+        if (tid < nr) s_base[tid] = v_base;
+        if (staged) {
+#pragma unroll
+            for (int q = 0; q < kHW; q++)
+                if ((uint64_t)q * kKBlock + tid < np + nn) s_words[q * kKBlock + tid] = v_w[q];
         }
         __syncthreads();
         auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
@@ -146,7 +163,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
         // a lane's windows are consecutive: its generating-pass tile advances at most once per gtile windows
         uint64_t gt = ~0ull, gnext = 0;
         for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon, uint32_t q) {
-            if (PHASE == 0 && H.gcnt) {
+            if (GC) {
                 const uint64_t wi = p0 + q;
                 if (gt == ~0ull) {
                     const uint64_t lt = (wi - origin) / H.gtile;
@@ -180,7 +197,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             const uint32_t v = (&s_bh[0][0])[j];
             if (v) atomicAdd(&bh[j], (unsigned long long)v);
         }
-        if (H.gcnt && w_hi > w_lo) {
+        if (GC && w_hi > w_lo) {
             // a tile whose windows all lie in this tile's is written; one shared
             // with a neighbour (the first and last) is added to
             const uint64_t g_hi = tbase + (w_hi - 1 - origin) / H.gtile;
@@ -333,7 +350,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     uint64_t *d_bounds = (uint64_t *)(hb + o_bd);
     MP_HIP(hipMemcpyAsync(hb + o_ids, chunk_first_id, 8 * (C + 1), hipMemcpyHostToDevice, ctx->stream));
     MP_HIP(hipMemcpyAsync(hb + o_t0, tile0.data(), 4 * (C + 1), hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(hist_tile_kernel<0>, dim3((unsigned)NT), dim3(kKBlock), 0, ctx->stream, P, H);
+    if (H.gcnt) hipLaunchKernelGGL((hist_tile_kernel<0, true>), dim3((unsigned)NT), dim3(kKBlock), 0, ctx->stream, P, H);
+    else hipLaunchKernelGGL((hist_tile_kernel<0, false>), dim3((unsigned)NT), dim3(kKBlock), 0, ctx->stream, P, H);
     MP_HIP(hipGetLastError());
     MP_TRY(scan_u32(ctx, H.tcnt, (uint64_t *)ctx->wg_off.p, M));
     hipLaunchKernelGGL(hist_bounds_kernel, dim3((unsigned)((nparts + 256) / 256)), dim3(256), 0, ctx->stream, H,

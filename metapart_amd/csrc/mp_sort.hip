@@ -322,25 +322,46 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         constexpr uint32_t kTab = 2 * 8 * kGenReads + 4 * (3 * kGenReads + 2);  // bytes of the read table
         const uint64_t mapo = (kTab + 8 * (np + nn) + 4 * nf + 7) & ~7ull;  // byte offset of the window map
         const bool staged = nr <= (uint32_t)kGenReads && mapo + kTile <= 8u * kTile;
-        uint64_t *s_rbase = s_buf, *s_rso = s_buf + kGenReads;
+        // staged, a read's first base and first sequence byte are 32-bit offsets
+        // from the staged words (bases from wn0 * 64, bytes from fw0 * 4), so the
+        // items index LDS with 32-bit arithmetic
+        uint32_t *s_rbase = (uint32_t *)s_buf, *s_rso = (uint32_t *)(s_buf + kGenReads);
         uint32_t *s_rpos = (uint32_t *)(s_buf + 2 * kGenReads), *s_rlen = s_rpos + kGenReads + 1,
                  *s_rlab = s_rlen + kGenReads;
         uint64_t *s_pw = s_buf + kTab / 8, *s_nw = s_pw + np;
         uint32_t *s_fw = (uint32_t *)(s_nw + nn);
         if (staged) {
-            for (uint32_t j = tid; j <= nr; j += kSB) {
-                s_rpos[j] = (uint32_t)(A.G.pos_off[ra + j] - pa);
-                if (j < nr) {
-                    s_rbase[j] = A.G.base_off[ra + j];
-                    s_rso[j] = A.G.seq_off[ra + j];
-                    s_rlen[j] = A.G.seq_len[ra + j];
-                    s_rlab[j] = label_of(A.G, ra + j);
-                }
-            }
-            for (uint32_t j = tid; j < np; j += kSB) s_pw[j] = A.G.packed[wp0 + j];
-            for (uint32_t j = tid; j < nn; j += kSB) s_nw[j] = A.G.nmask[wn0 + j];
+            // a lane's first entry of every array and its first kFQ FASTQ words
+            // are loaded together (clamped indices, no branch), then stored; what
+            // is left (long reads) follows in plain loops.  nr >= 1 (rb >= ra).
+            static_assert(kGenReads < kSB, "one table entry per lane");
+            constexpr int kFQ = 6;
             const uint32_t *fq32 = (const uint32_t *)A.G.fq;  // 4-aligned words inside the records
-            for (uint32_t j = tid; j < nf; j += kSB) s_fw[j] = fq32[fw0 + j];
+            const uint32_t jt = min((uint32_t)tid, nr), jr = min((uint32_t)tid, nr - 1);
+            const uint64_t t_pos = A.G.pos_off[ra + jt];
+            const uint64_t t_base = A.G.base_off[ra + jr], t_so = A.G.seq_off[ra + jr];
+            const uint32_t t_len = A.G.seq_len[ra + jr];
+            const uint64_t v_pw = np ? A.G.packed[wp0 + min((uint64_t)tid, np - 1)] : 0;
+            const uint64_t v_nw = nn ? A.G.nmask[wn0 + min((uint64_t)tid, nn - 1)] : 0;
+            uint32_t v_fw[kFQ];
+#pragma unroll
+            for (int q = 0; q < kFQ; q++) v_fw[q] = nf ? fq32[fw0 + min((uint64_t)(q * kSB + tid), nf - 1)] : 0;
+            const uint32_t t_lab = label_of(A.G, ra + jr);  // last: it branches on the scan
+            if ((uint32_t)tid <= nr) s_rpos[tid] = (uint32_t)(t_pos - pa);
+            if ((uint32_t)tid < nr) {
+                s_rbase[tid] = (uint32_t)(t_base - (wn0 << 6));
+                s_rso[tid] = (uint32_t)(t_so - (fw0 << 2));
+                s_rlen[tid] = t_len;
+                s_rlab[tid] = t_lab;
+            }
+            if ((uint64_t)tid < np) s_pw[tid] = v_pw;
+            if ((uint64_t)tid < nn) s_nw[tid] = v_nw;
+#pragma unroll
+            for (int q = 0; q < kFQ; q++)
+                if ((uint64_t)(q * kSB + tid) < nf) s_fw[q * kSB + tid] = v_fw[q];
+            for (uint32_t j = kSB + tid; j < np; j += kSB) s_pw[j] = A.G.packed[wp0 + j];
+            for (uint32_t j = kSB + tid; j < nn; j += kSB) s_nw[j] = A.G.nmask[wn0 + j];
+            for (uint32_t j = kFQ * kSB + tid; j < nf; j += kSB) s_fw[j] = fq32[fw0 + j];
         }
         __syncthreads();
         // window -> read of the tile (one byte each), so that the items need no search
@@ -371,16 +392,19 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 if constexpr (ST) return s_rpos[j];
                 else return (uint32_t)(A.G.pos_off[ra + j] - pa);
             };
+            // staged: w and x relative (bases from wn0 * 64, 2-bit word wp0 is
+            // word 2 * wn0 + dpw of that origin, bytes from fw0 * 4)
+            const uint32_t dpw = (uint32_t)(wp0 - 2 * wn0);
             auto nword = [&](uint64_t w) -> uint64_t {
-                if constexpr (ST) return s_nw[w - wn0];
+                if constexpr (ST) return s_nw[(uint32_t)w];
                 else return A.G.nmask[w];
             };
             auto pword = [&](uint64_t w) -> uint64_t {
-                if constexpr (ST) return s_pw[w - wp0];
+                if constexpr (ST) return s_pw[(uint32_t)w - dpw];
                 else return A.G.packed[w];
             };
             auto fqb = [&](uint64_t x) -> uint8_t {
-                if constexpr (ST) return (uint8_t)(s_fw[(x >> 2) - fw0] >> (8 * (x & 3)));
+                if constexpr (ST) return ((const uint8_t *)s_fw)[(uint32_t)x];
                 else return A.G.fq[x];
             };
             uint32_t lo = 0;  // read of the lane's last item: its items are 64 windows apart, so a few steps on
@@ -408,25 +432,25 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 }
                 const uint64_t r = ra + lo;
                 const uint32_t p = q - rpos(lo);
-                uint64_t rbase, so;
+                uint64_t b, x;  // the window's first base and its first sequence byte (staged: 32-bit offsets)
                 uint32_t L;
                 if constexpr (ST) {
-                    rbase = s_rbase[lo];
-                    so = s_rso[lo];
+                    b = (uint32_t)(s_rbase[lo] + p);
+                    x = (uint32_t)(s_rso[lo] + p);
                     L = s_rlen[lo];
                 } else {
-                    rbase = A.G.base_off[r];
-                    so = A.G.seq_off[r];
+                    b = A.G.base_off[r] + p;
+                    x = A.G.seq_off[r] + p;
                     L = A.G.seq_len[r];
                 }
                 uint64_t canon;
                 bool fwd;
-                if (!kmer_from(A.G.K, rbase + p, p, nword, pword, canon, fwd)) continue;
+                if (!kmer_from(A.G.K, b, p, nword, pword, canon, fwd)) continue;
                 const uint64_t bk = canon >> A.G.bshift;
                 if (bk < A.G.lo || bk >= A.G.hi) continue;
                 k[i] = canon;
-                const uint8_t prev = p == 0 ? (uint8_t)'N' : fqb(so + p - 1);
-                const uint8_t next = p == L - (uint32_t)A.G.K ? (uint8_t)'N' : fqb(so + p + A.G.K);
+                const uint8_t prev = p == 0 ? (uint8_t)'N' : fqb(x - 1);
+                const uint8_t next = p == L - (uint32_t)A.G.K ? (uint8_t)'N' : fqb(x + A.G.K);
                 const uint8_t a = adj_from(prev, next, fwd);
                 ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
                 if constexpr (ST) l[i] = s_rlab[lo];
@@ -434,22 +458,71 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 dr[i] = digit_of(k[i], a, P.shift) << 16;
             }
         };
-        if (staged) items(std::true_type{});
-        else items(std::false_type{});
+        if (staged) {
+            // Branch-free items from LDS, so that the compiler can overlap the
+            // items' LDS reads: every item reads in-bounds entries (one past the
+            // tile's windows takes the tile's first window and is dropped), the
+            // window's K bits are the top of a 64-bit funnel over two words
+            // (kmer_from's test and code, without its branches), and the
+            // neighbour bytes are read at the window itself where 'N' stands in.
+            const uint32_t dpw = (uint32_t)(wp0 - 2 * wn0), nwin = (uint32_t)(lim - base);
+            const int K = A.G.K;
+            const uint8_t *s_fb = (const uint8_t *)s_fw;
+#pragma unroll
+            for (int i = 0; i < kSI; i++) {
+                const uint32_t iw = (uint32_t)(wave * (kSI * kWave) + i * kWave) + lane;
+                const bool in = iw < nwin;
+                const uint32_t lo = s_map[in ? iw : 0u];
+                const uint32_t p = q0 + (in ? iw : 0u) - s_rpos[lo];
+                const uint32_t b = s_rbase[lo] + p, x = s_rso[lo] + p, L = s_rlen[lo];
+                const uint32_t o = b & 63, wn = b >> 6;
+                const uint64_t nx = (s_nw[wn] << o) | ((s_nw[wn + 1] >> 1) >> (63 - o));
+                const uint32_t o2 = 2 * (b & 31), wp = (b >> 5) - dpw;
+                const uint64_t px = (s_pw[wp] << o2) | ((s_pw[wp + 1] >> 1) >> (63 - o2));
+                uint64_t F = px >> (64 - 2 * K);
+                const uint64_t R = revcomp(F, K);
+                if (K == 32 && p > 0) F = 0;
+                const bool fwd = F < R;
+                const uint64_t canon = fwd ? F : R, bk = canon >> A.G.bshift;
+                const bool valid = in && (nx >> (64 - K)) == 0 && bk >= A.G.lo && bk < A.G.hi;
+                const bool first = p == 0, last = p == L - (uint32_t)K;
+                const uint8_t pc = s_fb[first ? x : x - 1], nc = s_fb[last ? x : x + K];
+                const uint8_t a = adj_from(first ? (uint8_t)'N' : pc, last ? (uint8_t)'N' : nc, fwd);
+                const uint32_t lab = s_rlab[lo];
+                k[i] = valid ? canon : 0;
+                l[i] = valid ? lab : 0;
+                ap[i / 4] |= (valid ? (uint32_t)a : 0u) << (8 * (i % 4));
+                dr[i] = valid ? digit_of(canon, a, P.shift) << 16 : (uint32_t)kRadix << 16;
+            }
+        } else {
+            items(std::false_type{});
+        }
     } else {
+        // every item's loads first, so that a lane's loads are all in flight
+        // together; then the digits (the sample pass's digit is a search loop,
+        // which would otherwise hold each item's loads back until the previous
+        // item's digit is known)
+        uint32_t av[kSI];
 #pragma unroll
         for (int i = 0; i < kSI; i++) {
             const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+            k[i] = 0;
+            l[i] = 0;
+            av[i] = 0;
             if (idx < lim) {
                 k[i] = P.keys_in[idx];
                 l[i] = P.lab_in[idx];
-                const uint8_t a = P.adj_in[idx];
-                ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
-                dr[i] = (P.shift == -2 ? group_of(grp_first, P.n_grp, l[i]) : digit_of(k[i], a, P.shift)) << 16;
-            } else {
-                k[i] = 0;
-                l[i] = 0;
-                dr[i] = (uint32_t)kRadix << 16;  // invalid
+                av[i] = P.adj_in[idx];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kSI; i++) {
+            const uint64_t idx = base + (uint64_t)wave * (kSI * kWave) + (uint64_t)i * kWave + lane;
+            dr[i] = (uint32_t)kRadix << 16;  // invalid
+            if (idx < lim) {
+                ap[i / 4] |= av[i] << (8 * (i % 4));
+                dr[i] = (P.shift == -2 ? group_of(grp_first, P.n_grp, l[i]) : digit_of(k[i], (uint8_t)av[i], P.shift))
+                        << 16;
             }
         }
     }
@@ -568,15 +641,29 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         s_gbase[tid] = P.digit_base[tid] + prefix - ds;  // output index of tile position 0 of this digit
     }
     __syncthreads();
+    // tile position j = i * kSB + tid goes to output index gp[i]; the loops
+    // have a fixed trip count and read LDS without a branch (j < kTile: the
+    // reads stay in the arrays; past nvalid they are not used), so that a
+    // lane's LDS reads are all issued before its stores (a loop over j waits
+    // on two dependent LDS reads per step)
     const uint32_t nvalid = s_nvalid;
-    if (P.dig_out) {
-        for (uint32_t j = tid; j < nvalid; j += kSB) {
-            const uint64_t g = s_gbase[s_dig[j]] + j, key = s_buf[j];
-            P.keys_out[g] = key;
-            P.dig_out[g] = (uint8_t)(key >> P.next_shift);
+    uint64_t gp[kSI];
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        const uint32_t j = i * kSB + tid;
+        gp[i] = s_gbase[s_dig[j]] + j;
+    }
+    {
+        uint64_t kv[kSI];  // every read before the first store: the stores might alias LDS for all the compiler knows
+#pragma unroll
+        for (int i = 0; i < kSI; i++) kv[i] = s_buf[i * kSB + tid];
+#pragma unroll
+        for (int i = 0; i < kSI; i++) {
+            if (i * kSB + tid < (int)nvalid) {
+                P.keys_out[gp[i]] = kv[i];
+                if (P.dig_out) P.dig_out[gp[i]] = (uint8_t)(kv[i] >> P.next_shift);
+            }
         }
-    } else {
-        for (uint32_t j = tid; j < nvalid; j += kSB) P.keys_out[s_gbase[s_dig[j]] + j] = s_buf[j];
     }
     __syncthreads();
     uint32_t *s_lab = (uint32_t *)s_buf;
@@ -589,10 +676,18 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         }
     }
     __syncthreads();
-    for (uint32_t j = tid; j < nvalid; j += kSB) {
-        const uint64_t g = s_gbase[s_dig[j]] + j;
-        P.lab_out[g] = s_lab[j];
-        P.adj_out[g] = s_adj[j];
+    uint32_t lv[kSI], av[kSI];
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        lv[i] = s_lab[i * kSB + tid];
+        av[i] = s_adj[i * kSB + tid];
+    }
+#pragma unroll
+    for (int i = 0; i < kSI; i++) {
+        if (i * kSB + tid < (int)nvalid) {
+            P.lab_out[gp[i]] = lv[i];
+            P.adj_out[gp[i]] = (uint8_t)av[i];
+        }
     }
 }
 
@@ -1237,7 +1332,7 @@ __device__ __forceinline__ bool group_start(const LocalPass &L, uint64_t i)
     return L.nseg && next_seg_start(L.seg, L.nseg, i) == i;
 }
 
-__global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
+__global__ __launch_bounds__(kLThreads, 8) void local_rank_kernel(LocalPass L)
 {
     // keys of [w0, w0 + kLCap) relative to w0, then the order keys of the groups
     __shared__ uint64_t s_key[kLCap];
@@ -1250,7 +1345,20 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     // the window and a margin after it, in one coalesced load; group starts
     // from LDS (the first position of the array is a start)
     const uint32_t nld = (uint32_t)(min(w1 + kLMarg, lim) - w0);
-    for (uint32_t j = threadIdx.x; j < nld; j += kLThreads) s_key[j] = L.keys[w0 + j];
+    {
+        // a fixed trip count and no branch (positions past nld load the last
+        // one, nld >= 1), so that a lane's loads are all issued before its stores
+        constexpr int kLd = (kLWin + kLMarg) / kLThreads;
+        static_assert(kLd * kLThreads == kLWin + kLMarg, "window load");
+        uint64_t v[kLd];
+#pragma unroll
+        for (int q = 0; q < kLd; q++) v[q] = L.keys[w0 + min((uint32_t)(q * kLThreads + threadIdx.x), nld - 1)];
+#pragma unroll
+        for (int q = 0; q < kLd; q++) {
+            const uint32_t j = q * kLThreads + threadIdx.x;
+            if (j < nld) s_key[j] = v[q];
+        }
+    }
     if (threadIdx.x == 0) {
         s_prev = w0 ? L.keys[w0 - 1] : 0;
         s_r1 = lim;
@@ -1335,7 +1443,7 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
     // its label and adj before the barrier and is written after it.  The
     // workgroup owns its groups' positions; its neighbours read keys there
     // only for their high bits, which a permutation inside a group keeps.
-    uint32_t rk[kQ], lb[kQ];  // rank | adj << 24 (~0: stays), label
+    uint32_t rk[kQ], lb[kQ], ab[kQ];  // rank (~0: stays), label, adj: the loads wait for nothing before the writes
 #pragma unroll
     for (int q = 0; q < kQ; q++) rk[q] = ~0u;
 #pragma unroll
@@ -1360,7 +1468,8 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
         uint32_t rank = gs;
         for (const uint64_t *sp = s_key + gs, *se = s_key + ge; sp < se; sp++) rank += *sp < cj;
         if (rank != p) {
-            rk[q] = rank | ((uint32_t)L.adj[w0 + p] << 24);  // rank < kLCap
+            rk[q] = rank;
+            ab[q] = L.adj[w0 + p];
             lb[q] = L.lab[w0 + p];
         }
     }
@@ -1369,10 +1478,10 @@ __global__ __launch_bounds__(kLThreads) void local_rank_kernel(LocalPass L)
 #pragma unroll
     for (int q = 0; q < kQ; q++) {
         if (rk[q] == ~0u) continue;
-        const uint64_t dst = w0 + (rk[q] & 0xFFFFFFu);
+        const uint64_t dst = w0 + rk[q];
         L.keys[dst] = kr[q];
         L.lab[dst] = lb[q];
-        L.adj[dst] = (uint8_t)(rk[q] >> 24);
+        L.adj[dst] = (uint8_t)ab[q];
         nmoved++;
     }
     // one plain store per wave (no barrier, no atomic: millions of adds on a
