@@ -490,8 +490,13 @@ __global__ __launch_bounds__(256) void union_kernel(GParams G, const uint64_t *_
             b[u] = t < G.n && join ? lb[u] : 0;
             live[u] = t < G.n && join;
         }
+        // the BAD words of every lane's labels loaded together (b = 0 when not
+        // joined: word 0, in bounds), not one after another behind live[u]
+        uint64_t bw[kUW];
 #pragma unroll
-        for (int u = 0; u < kUW; u++) live[u] = live[u] && !is_bad(badbits, b[u]);
+        for (int u = 0; u < kUW; u++) bw[u] = badbits[b[u] >> 6];
+#pragma unroll
+        for (int u = 0; u < kUW; u++) live[u] = live[u] && !((bw[u] >> (b[u] & 63)) & 1);
 #pragma unroll
         for (int u = 0; u < kUW; u++) {
             uint64_t cand = __ballot(live[u]) & lt;

@@ -158,11 +158,9 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
                 if ((uint64_t)q * kKBlock + tid < np + nn) s_words[q * kKBlock + tid] = v_w[q];
         }
         __syncthreads();
-        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
-        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
         // a lane's windows are consecutive: its generating-pass tile advances at most once per gtile windows
         uint64_t gt = ~0ull, gnext = 0;
-        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon, uint32_t q) {
+        auto body = [&](uint64_t canon, uint32_t q) {
             if (GC) {
                 const uint64_t wi = p0 + q;
                 if (gt == ~0ull) {
@@ -187,7 +185,19 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
                 for (int y = 0; y < 8; y++)
                     if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
             }
-        });
+        };
+        // the words from LDS or from memory in two copies of the loop: one
+        // accessor choosing at run time reads both through flat loads, whose
+        // waits also wait for the items' stores
+        if (staged) {
+            for_group_windows(
+                P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) -> uint64_t { return s_words[np + (w - wn0)]; },
+                [&](uint64_t w) -> uint64_t { return s_words[w - wp0]; }, body);
+        } else {
+            for_group_windows(
+                P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) -> uint64_t { return P.nmask[w]; },
+                [&](uint64_t w) -> uint64_t { return P.packed[w]; }, body);
+        }
     }
     if (PHASE == 0) {
         __syncthreads();
@@ -656,15 +666,22 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
             for (uint32_t j = tid; j < nn; j += kKBlock) s_words[np + j] = P.nmask[wn0 + j];
         }
         __syncthreads();
-        auto nword = [&](uint64_t w) -> uint64_t { return staged ? s_words[np + (w - wn0)] : P.nmask[w]; };
-        auto pword = [&](uint64_t w) -> uint64_t { return staged ? s_words[w - wp0] : P.packed[w]; };
-        for_group_windows(P.K, s_pref, s_base, nr, tid, kKBlock, nword, pword, [&](uint64_t canon, uint32_t) {
+        auto body = [&](uint64_t canon, uint32_t) {
             const uint64_t bk = canon >> P.bshift;
             if (bk < P.lo || bk >= P.hi) return;
 #pragma unroll
             for (int b = 0; b < 8; b++)
                 if ((pass_mask >> (b + 1)) & 1) atomicAdd(&h[b][(canon >> (8 * b)) & 0xFF], 1u);
-        });
+        };
+        if (staged) {  // two copies of the loop, as in hist_tile_kernel
+            for_group_windows(
+                P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) -> uint64_t { return s_words[np + (w - wn0)]; },
+                [&](uint64_t w) -> uint64_t { return s_words[w - wp0]; }, body);
+        } else {
+            for_group_windows(
+                P.K, s_pref, s_base, nr, tid, kKBlock, [&](uint64_t w) -> uint64_t { return P.nmask[w]; },
+                [&](uint64_t w) -> uint64_t { return P.packed[w]; }, body);
+        }
     }
     __syncthreads();
     for (int j = tid; j < 8 * kSortRadix; j += kKBlock) {
