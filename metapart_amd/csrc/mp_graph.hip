@@ -1216,7 +1216,9 @@ int filter_union_groups(mp_ctx *ctx, uint64_t n, const uint64_t *seg_pos, int ns
 // ---------------------------------------------------------------- finish
 // Sequential pair step (unionFindPeReads, metapart.c:2801-2867) on the host,
 // for the pairs whose outcome depends on the order.
-constexpr uint64_t kPairAhead = 16;  // pairs between a prefetch and its use
+// pairs between the prefetch of a pair's nodes and their use, and between the
+// prefetch of those nodes' parents (read from the now cached nodes) and their use
+constexpr uint64_t kPairAhead = 16, kParentAhead = 8;
 
 struct PairNode {  // a compact root: its parent, and its component's size while it is a root
     uint32_t p, s;
@@ -1369,6 +1371,11 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
                         const uint64_t f = hpairs[q + kPairAhead];
                         __builtin_prefetch(&nd[(uint32_t)f]);
                         __builtin_prefetch(&nd[(uint32_t)(f >> 32)]);
+                    }
+                    if (q + kParentAhead < qe) {  // and their parents, the second step of its finds
+                        const uint64_t f = hpairs[q + kParentAhead];
+                        __builtin_prefetch(&nd[nd[(uint32_t)f].p]);
+                        __builtin_prefetch(&nd[nd[(uint32_t)(f >> 32)].p]);
                     }
                     const uint32_t a = hfind(nd.data(), (uint32_t)hpairs[q]),
                                    b = hfind(nd.data(), (uint32_t)(hpairs[q] >> 32));
