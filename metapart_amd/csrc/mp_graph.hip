@@ -188,6 +188,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     __shared__ uint64_t s_pl[kFWords][6];  // bits 0-2 of the in class, then of the out class
     __shared__ uint16_t s_vd[kFTile];      // verdicts at the tile's run starts
     __shared__ uint16_t s_list[kFTile];    // the tile's run starts
+    __shared__ uint8_t s_adj[kFStage];     // the tuples' adj bytes
     __shared__ uint64_t s_seg[kFSeg + 1];
     __shared__ uint32_t s_n;
     const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
@@ -210,9 +211,6 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         s0 = sg[lo];
         s1 = sg[lo + 1];
     };
-    static_assert(kFIter == 9, "adj bytes kept in two registers");
-    uint64_t ap0 = 0;  // this lane's adj bytes, for step 3 (the ninth in ap1)
-    uint32_t ap1 = 0;
     uint32_t ev = 0, z = 0;  // runs evaluated (the last run of a segment is not), runs zeroed
     // each iteration's key, previous key and adj byte are loaded one iteration ahead
     uint64_t nk = 0, nkp = 0;
@@ -244,8 +242,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
             st = i == s0 || kp != k;
             nz = k != 0;
         }
-        if (q < 8) ap0 |= (uint64_t)a << (8 * q);
-        else ap1 = a;
+        s_adj[j] = (uint8_t)a;  // for step 3 (in LDS: registers are what the run loop lacks)
         const uint64_t mst = __ballot(st), mnz = __ballot(nz);
         uint64_t pl[6];
 #pragma unroll
@@ -294,18 +291,26 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         if (e != s1 || (G.open_last && e == G.n)) {  // the last run of a segment is never evaluated
             ev++;
             if (e <= t0 + cnt_s) {
-                uint32_t in[8], out[8];
+                // in and out tallies of class c in the halves of io[c] (a run here
+                // holds at most kFStage < 2^16 tuples): half the registers, which
+                // keeps the run loop from spilling
+                uint32_t io[7];
 #pragma unroll
-                for (int c = 0; c < 8; c++) in[c] = out[c] = 0;
+                for (int c = 0; c < 7; c++) io[c] = 0;
                 for (uint32_t wd = j / kWave; wd * kWave < je; wd++) {
                     const uint32_t lo = max(j, wd * kWave) - wd * kWave, hi = min(je, (wd + 1) * kWave) - wd * kWave;
                     const uint64_t m = (hi == 64 ? ~0ull : ((1ull << hi) - 1)) & ~((1ull << lo) - 1);
 #pragma unroll
-                    for (int c = 0; c < 7; c++) {
-                        in[c] += class_count(s_pl[wd], m, c);
-                        out[c] += class_count(s_pl[wd] + 3, m, c);
-                    }
+                    for (int c = 0; c < 7; c++)
+                        io[c] += class_count(s_pl[wd], m, c) | (class_count(s_pl[wd] + 3, m, c) << 16);
                 }
+                uint32_t in[8], out[8];
+#pragma unroll
+                for (int c = 0; c < 7; c++) {
+                    in[c] = io[c] & 0xFFFF;
+                    out[c] = io[c] >> 16;
+                }
+                in[7] = out[7] = 0;
                 v = run_verdict(in, out, e - i);
                 if (v & kVZero) {
                     zero_run(G, i, e);
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         const uint32_t c = s_vd[sj];
         if (!(c & kVMark)) continue;
         const uint32_t v = (c & 0x1F) | (((c >> 5) & 7) << 8) | (((c >> 8) & 7) << 12);
-        if (tuple_filtered(v, q < 8 ? (uint32_t)(ap0 >> (8 * q)) & 0xFF : ap1)) fm |= 1u << q;
+        if (tuple_filtered(v, s_adj[j])) fm |= 1u << q;
     }
     uint32_t lv[kFIter];
 #pragma unroll
