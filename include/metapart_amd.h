@@ -72,7 +72,9 @@ typedef struct {
     int k;                /* k-mer length, 11..32 */
     int min3;             /* 0: every k-mer; 1: three window minimizers per read */
     uint64_t hbm_budget;  /* device bytes this ctx may hold; 0 = 90 % of free memory */
-    int sort_digit_bits;  /* radix digit width: 0 or 8 (the only width; anything else is MP_EINVAL) */
+    int sort_digit_bits;  /* scans' radix digits: 8 = bytes; 0 or 7 = bytes, except that at k = 29..31 the
+                             k-mer's bits above bit 48 are split evenly over the last two passes (k = 31:
+                             7 + 7 instead of 8 + 6); anything else is MP_EINVAL */
     int sort_tile;        /* tuples per radix tile: 0 (1024 lanes x 6), 3072, 4096, 6144 (512 lanes) or 8192 */
     int sort_graph;       /* 1: a radix pass chain of counted passes only (a batch's) is captured in a HIP graph
                              and replayed while the sort's shape and buffers repeat; 0: launched one by one */

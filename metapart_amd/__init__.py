@@ -192,10 +192,10 @@ class Context:
     """One device context (mp_ctx)."""
 
     def __init__(self, k: int, min3: bool = False, device: int = 0, hbm_budget: int = 0, sort_tile: int = 0,
-                 sort_graph: bool = False):
+                 sort_graph: bool = False, sort_digit_bits: int = 0):
         self.k = k
         self.min3 = bool(min3)
-        cfg = Cfg(device=device, k=k, min3=1 if min3 else 0, hbm_budget=hbm_budget, sort_digit_bits=0,
+        cfg = Cfg(device=device, k=k, min3=1 if min3 else 0, hbm_budget=hbm_budget, sort_digit_bits=sort_digit_bits,
                   sort_tile=sort_tile, sort_graph=1 if sort_graph else 0)
         h = ct.c_void_p()
         rc = lib().mp_ctx_create(ct.byref(cfg), ct.byref(h))

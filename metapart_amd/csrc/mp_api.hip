@@ -19,7 +19,8 @@ int mp_ctx_create(const mp_cfg *cfg, mp_ctx **out)
     if (!cfg || !out) return MP_EINVAL;
     *out = nullptr;
     if (cfg->k <= 10 || cfg->k > 32) return MP_EINVAL;
-    if (cfg->sort_digit_bits != 0 && cfg->sort_digit_bits != 8) return MP_EINVAL;  // 8-bit digits only
+    // 8: byte digits; 0 or 7: the scans split the k-mer's top bits over their last two passes
+    if (cfg->sort_digit_bits != 0 && cfg->sort_digit_bits != 7 && cfg->sort_digit_bits != 8) return MP_EINVAL;
     if (cfg->sort_tile != 0 && cfg->sort_tile != 3072 && cfg->sort_tile != 4096 && cfg->sort_tile != 6144 &&
         cfg->sort_tile != 8192)
         return MP_EINVAL;

@@ -241,6 +241,21 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
 struct GenSource;
 // passes whose digits the sort of a scan's tuples reads (bit p = radix pass p)
 uint32_t scan_pass_mask(int K);
+// The k-mer digits of the radix passes: pass 1 + b takes bits [shift[b],
+// shift[b] + bits[b]) of the key.  Bytes by default.  A scan's layout may
+// split the k-mer's bits above bit 48 evenly over the last two passes (k = 31:
+// 7 + 7 bits instead of 8 + 6): fewer digits per pass keep a tile's digit runs
+// longer, and its partial cache lines fewer.
+struct KeyLayout {
+    uint8_t shift[8], bits[8];
+    __host__ __device__ __forceinline__ uint32_t digit(uint64_t k, int b) const
+    {
+        return (uint32_t)(k >> shift[b]) & ((1u << bits[b]) - 1u);
+    }
+};
+KeyLayout byte_layout();
+// the layout of the ctx's scans (cfg.sort_digit_bits: 8 bytes; 0 or 7 the split top)
+KeyLayout scan_layout(const mp_ctx *ctx);
 // kSortPasses x kSortRadix digit counts of the scan's tuples (host, passes of
 // pass_mask) and their number; fills *g for radix_sort_tuples.  all: the scan
 // holds every k-mer, so the counts of the last m-mer histogram serve.
@@ -265,9 +280,11 @@ uint64_t sort_tile_items(const mp_ctx *ctx);
 // n_grp + 1 tuple offsets of the samples).  *cur selects the buffer holding the data.
 // With gen, the first pass generates the tuples (gen's histograms replace the
 // histogram read) into the buffer *cur does not select.
+// Key passes take their digits from *kl (nullptr: bytes).
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base, const GenSource *gen = nullptr, const SortSegments *segs = nullptr);
+                       uint64_t *grp_base, const GenSource *gen = nullptr, const SortSegments *segs = nullptr,
+                       const KeyLayout *kl = nullptr);
 int iota_u32(mp_ctx *ctx, uint32_t *p, uint64_t n);
 int filter_union_scan(mp_ctx *ctx, uint64_t n, const uint64_t *seg_bounds, int T, mp_scan_stats *st);
 // Same over segments at the given tuple positions (nseg + 1, host) with

@@ -60,8 +60,9 @@ constexpr int kHWords = 1024;            // staged stream words of a group of kR
 constexpr int kBhRep = 32;  // copies of the byte histograms (tile t adds to copy t % kBhRep)
 
 struct HParams {
-    unsigned long long *bh;     // kBhRep x 8 x 256 counts of the k-mer bytes in bmask
-    uint32_t bmask;             // bit 1 + b: count byte b (the radix passes of the scans)
+    unsigned long long *bh;     // kBhRep x 8 x 256 counts of the k-mer digits in bmask
+    uint32_t bmask;             // bit 1 + b: count digit b (the radix passes of the scans)
+    KeyLayout kl;               // the scans' key digits
     const uint64_t *chunk_ids;  // C+1 first read ids
     const uint32_t *tile0;      // C+1 first tile of each chunk
     int C;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             if (PHASE == 0) {  // digit counts of a later sort over every k-mer
 #pragma unroll
                 for (int y = 0; y < 8; y++)
-                    if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][(canon >> (8 * y)) & 0xFF], 1u);
+                    if ((H.bmask >> (y + 1)) & 1) atomicAdd(&s_bh[y][H.kl.digit(canon, y)], 1u);
             }
         };
         // the words from LDS or from memory in two copies of the loop: one
@@ -303,6 +304,7 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     MP_TRY(fill_u32(ctx, ctx->khist.p, 0, 2 * ncopies * 8 * kSortRadix));
     H.bh = (unsigned long long *)ctx->khist.p;
     H.bmask = scan_pass_mask(ctx->cfg.k) & 0x1FEu;
+    H.kl = scan_layout(ctx);
     H.per_chunk = seg ? 1 : 0;
     ctx->khist_copies = (int)ncopies;
     // digit counts per generating-pass tile of the lowest sorted byte, for a
@@ -346,7 +348,7 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
             MP_TRY(fill_u32(ctx, ctx->gcnt.p, 0, gtiles * kSortRadix));
             H.gcnt = (uint32_t *)ctx->gcnt.p;
             H.gtile = gtile;
-            H.gshift = 8 * (__builtin_ctz(H.bmask) - 1);
+            H.gshift = H.kl.shift[__builtin_ctz(H.bmask) - 1];  // the generating pass's digit: a whole byte
             ctx->gcnt_tile = gtile;
             ctx->gcnt_tiles = gtiles;
             ctx->gcnt_byte = __builtin_ctz(H.bmask) - 1;
@@ -640,7 +642,7 @@ int count_chunks_device(mp_ctx *ctx, const uint64_t *chunk_first_id, int C, uint
 // generates them: every in-range k-mer of every read, counted per k-mer byte
 // of the pass mask in LDS (groups of kRG reads, grid-stride), then added to
 // ghist[pass][digit] (pass 1 + b for byte b).
-__global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uint32_t pass_mask,
+__global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uint32_t pass_mask, KeyLayout kl,
                                                                   unsigned long long *__restrict__ ghist)
 {
     __shared__ uint32_t h[8][kSortRadix];
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_digit_hist_kernel(KParams P, uin
             if (bk < P.lo || bk >= P.hi) return;
 #pragma unroll
             for (int b = 0; b < 8; b++)
-                if ((pass_mask >> (b + 1)) & 1) atomicAdd(&h[b][(canon >> (8 * b)) & 0xFF], 1u);
+                if ((pass_mask >> (b + 1)) & 1) atomicAdd(&h[b][kl.digit(canon, b)], 1u);
         };
         if (staged) {  // two copies of the loop, as in hist_tile_kernel
             for_group_windows(
@@ -814,7 +816,8 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
         const uint64_t ngrp = (ctx->N + kRG - 1) / kRG;
         const unsigned blocks = (unsigned)(ngrp < 4096 ? ngrp : 4096);
         if (blocks) {
-            hipLaunchKernelGGL(kmer_digit_hist_kernel, dim3(blocks), dim3(kKBlock), 0, ctx->stream, P, pass_mask, gh);
+            hipLaunchKernelGGL(kmer_digit_hist_kernel, dim3(blocks), dim3(kKBlock), 0, ctx->stream, P, pass_mask,
+                               scan_layout(ctx), gh);
             MP_HIP(hipGetLastError());
         }
         MP_HIP(hipMemcpyAsync(g->h, gh, sizeof(g->h), hipMemcpyDeviceToHost, ctx->stream));

@@ -123,6 +123,7 @@ struct SortPass {
     uint8_t *adj_out;
     uint64_t n;
     int shift;  // -1: the digit is the adj byte, -2: the sample of the label
+    uint32_t mask;  // a key digit: (key >> shift) & mask
     const uint64_t *grp_first;  // n_grp + 1 first read ids of the samples
     int n_grp;
     const uint64_t *digit_base;
@@ -132,6 +133,7 @@ struct SortPass {
     const uint64_t *offs;  // counted passes: output index of each (tile, digit)'s first tuple, tile-major
     uint8_t *dig_out;      // if set: the next pass's digit of every tuple, at its output index
     int next_shift;
+    uint32_t next_mask;
     int xcd_tiles;         // counted passes: consecutive tiles on one XCD (workgroups go round-robin to 8 XCDs)
     SegTiles seg;             // the input's tiles (scatter passes): per sample when segmented
     const uint32_t *chk_cnt;  // counted generating pass: the per-tile counts its offsets came from, checked
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
                                                            const uint8_t *__restrict__ adj,
                                                            const uint32_t *__restrict__ lab,
                                                            const uint64_t *__restrict__ grp_first, int n_grp,
-                                                           uint64_t n, uint32_t pass_mask,
+                                                           uint64_t n, uint32_t pass_mask, KeyLayout kl,
                                                            unsigned long long *__restrict__ ghist)
 {
     __shared__ uint32_t h[kPasses][kRadix];
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
             const uint64_t k = keys[i];
 #pragma unroll
             for (int b = 0; b < 8; b++)
-                if ((pass_mask >> (1 + b)) & 1) atomicAdd(&h[1 + b][(k >> (8 * b)) & 0xFF], 1u);
+                if ((pass_mask >> (1 + b)) & 1) atomicAdd(&h[1 + b][kl.digit(k, b)], 1u);
         }
         if (grp_first) atomicAdd(&h[9][group_of(s_first, n_grp, lab[i])], 1u);
     }
@@ -192,9 +194,9 @@ __global__ __launch_bounds__(kRadix) void sort_hist_kernel(const uint64_t *__res
     }
 }
 
-__device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift)
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, uint8_t a, int shift, uint32_t mask)
 {
-    return shift < 0 ? (uint32_t)a : (uint32_t)((k >> shift) & 0xFF);
+    return shift < 0 ? (uint32_t)a : (uint32_t)(k >> shift) & mask;
 }
 
 // The generating pass: its items are the reads' k-mer windows, tile t
@@ -455,7 +457,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 ap[i / 4] |= (uint32_t)a << (8 * (i % 4));
                 if constexpr (ST) l[i] = s_rlab[lo];
                 else l[i] = label_of(A.G, r);
-                dr[i] = digit_of(k[i], a, P.shift) << 16;
+                dr[i] = digit_of(k[i], a, P.shift, P.mask) << 16;
             }
         };
         if (staged) {
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
                 k[i] = valid ? canon : 0;
                 l[i] = valid ? lab : 0;
                 ap[i / 4] |= (valid ? (uint32_t)a : 0u) << (8 * (i % 4));
-                dr[i] = valid ? digit_of(canon, a, P.shift) << 16 : (uint32_t)kRadix << 16;
+                dr[i] = valid ? digit_of(canon, a, P.shift, P.mask) << 16 : (uint32_t)kRadix << 16;
             }
         } else {
             items(std::false_type{});
@@ -521,7 +523,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
             dr[i] = (uint32_t)kRadix << 16;  // invalid
             if (idx < lim) {
                 ap[i / 4] |= av[i] << (8 * (i % 4));
-                dr[i] = (P.shift == -2 ? group_of(grp_first, P.n_grp, l[i]) : digit_of(k[i], (uint8_t)av[i], P.shift))
+                dr[i] = (P.shift == -2 ? group_of(grp_first, P.n_grp, l[i]) : digit_of(k[i], (uint8_t)av[i], P.shift, P.mask))
                         << 16;
             }
         }
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(kSB, (kSB == 512 && kSI <= 6) ? 8 : 2 * kSB / 256) 
         for (int i = 0; i < kSI; i++) {
             if (i * kSB + tid < (int)nvalid) {
                 P.keys_out[gp[i]] = kv[i];
-                if (P.dig_out) P.dig_out[gp[i]] = (uint8_t)(kv[i] >> P.next_shift);
+                if (P.dig_out) P.dig_out[gp[i]] = (uint8_t)((kv[i] >> P.next_shift) & P.next_mask);
             }
         }
     }
@@ -764,7 +766,7 @@ __global__ __launch_bounds__(256) void digit_count_kernel(const uint8_t *__restr
 
 // the same digit counts from the keys themselves (byte `shift / 8` of each)
 __global__ __launch_bounds__(256) void key_digit_count_kernel(const uint64_t *__restrict__ keys, uint64_t n, int shift,
-                                                              uint32_t tile, uint64_t tiles, SegTiles S,
+                                                              uint32_t mask, uint32_t tile, uint64_t tiles, SegTiles S,
                                                               uint32_t *__restrict__ cnt, uint32_t *__restrict__ csum)
 {
     __shared__ uint32_t h[kRadix];
@@ -775,7 +777,7 @@ __global__ __launch_bounds__(256) void key_digit_count_kernel(const uint64_t *__
         __syncthreads();
         uint64_t a, e;
         tile_range(S, t, tile, n, a, e);
-        for (uint64_t i = a + threadIdx.x; i < e; i += 256) atomicAdd(&h[(keys[i] >> shift) & 0xFF], 1u);
+        for (uint64_t i = a + threadIdx.x; i < e; i += 256) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & mask], 1u);
         __syncthreads();
         const uint32_t v = h[threadIdx.x];
         cnt[t * kRadix + threadIdx.x] = v;
@@ -902,10 +904,18 @@ uint64_t sort_tile_items(const mp_ctx *ctx)
     return (uint64_t)SB * SI;
 }
 
+static uint64_t kl_id(const KeyLayout &L)
+{
+    uint64_t v = 0;
+    for (int b = 0; b < 8; b++) v = v * 131 + L.shift[b] * 16 + L.bits[b];
+    return v;
+}
+
 int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj, int *cur, uint64_t n,
                        uint32_t pass_mask, int *passes_out, const uint64_t *d_grp_first, int n_grp,
-                       uint64_t *grp_base, const GenSource *gen, const SortSegments *segs)
+                       uint64_t *grp_base, const GenSource *gen, const SortSegments *segs, const KeyLayout *kl)
 {
+    const KeyLayout KL = kl ? *kl : byte_layout();
     if (segs && segs->ns > 0 && (!gen || !gen->gcnt || !gen->gcnt_seg))
         return ctx->fail(MP_EINVAL, "a segmented generated sort needs the per-sample tile counts");
     *passes_out = 0;
@@ -921,7 +931,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL(sort_hist_kernel, dim3((unsigned)blocks), dim3(kRadix), 0, ctx->stream,
                            (const uint64_t *)keys[*cur].p, (const uint8_t *)adj[*cur].p,
-                           (const uint32_t *)lab[*cur].p, d_grp_first, n_grp, n, pass_mask, gh);
+                           (const uint32_t *)lab[*cur].p, d_grp_first, n_grp, n, pass_mask, KL, gh);
         MP_HIP(hipGetLastError());
     }
     hipEventRecord(ctx->pev[1], ctx->stream);
@@ -1129,7 +1139,8 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.lab_out = (uint32_t *)lab[dst].p;
         P.adj_out = (uint8_t *)adj[dst].p;
         P.n = n;
-        P.shift = p == 0 ? -1 : p == 9 ? -2 : 8 * (p - 1);
+        P.shift = p == 0 ? -1 : p == 9 ? -2 : (int)KL.shift[p - 1];
+        P.mask = p >= 1 && p <= 8 ? (1u << KL.bits[p - 1]) - 1u : 0xFFu;
         P.grp_first = d_grp_first;
         P.n_grp = n_grp;
         P.digit_base = d_base + p * kRadix;
@@ -1138,7 +1149,8 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
         P.epoch = ctx->epoch;
         P.offs = d_offs;
         P.dig_out = next_counted ? d_dig : nullptr;
-        P.next_shift = next_counted ? 8 * (q - 1) : 0;
+        P.next_shift = next_counted ? (int)KL.shift[q - 1] : 0;
+        P.next_mask = next_counted ? (1u << KL.bits[q - 1]) - 1u : 0xFFu;
         P.xcd_tiles = xcd ? 1 : 0;
         P.seg = segT;
         P.chk_cnt = p == gen_p && gen_offs ? gen->gcnt : nullptr;
@@ -1147,8 +1159,8 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
             if (timing) hipEventRecord(ctx->cev[executed], ctx->stream);
             if (count_keys || from_keys)
                 hipLaunchKernelGGL(key_digit_count_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
-                                   (const uint64_t *)keys[src].p, n, 8 * (p - 1), (uint32_t)kTile, tiles, segT,
-                                   d_cnt, d_csum);
+                                   (const uint64_t *)keys[src].p, n, (int)KL.shift[p - 1],
+                                   (1u << KL.bits[p - 1]) - 1u, (uint32_t)kTile, tiles, segT, d_cnt, d_csum);
             else
                 hipLaunchKernelGGL(digit_count_kernel, dim3((unsigned)nchunks), dim3(256), 0, ctx->stream,
                                    (const uint8_t *)d_dig, n, (uint32_t)kTile, tiles, segT, d_cnt, d_csum);
@@ -1196,7 +1208,7 @@ int radix_sort_generic(mp_ctx *ctx, DevArray *keys, DevArray *lab, DevArray *adj
     }
     if (graphable) {
         const uint64_t key[] = {n, tiles, gtiles, (uint64_t)SB, (uint64_t)SI, (uint64_t)ns, (uint64_t)gen_p,
-                                (uint64_t)pass_mask, (uint64_t)cur0, (uint64_t)xcd, (uint64_t)(uintptr_t)keys[0].p,
+                                (uint64_t)pass_mask, (uint64_t)cur0, (uint64_t)xcd, kl_id(KL), (uint64_t)(uintptr_t)keys[0].p,
                                 (uint64_t)(uintptr_t)keys[1].p, (uint64_t)(uintptr_t)lab[0].p,
                                 (uint64_t)(uintptr_t)lab[1].p, (uint64_t)(uintptr_t)adj[0].p,
                                 (uint64_t)(uintptr_t)adj[1].p, (uint64_t)(uintptr_t)d_dig, (uint64_t)(uintptr_t)d_cnt,
@@ -1538,6 +1550,34 @@ static int local_bits(int K)
 
 static int sort_local_bits(int K) { return getenv("MP_SORT_GLOBAL") ? 0 : local_bits(K); }
 
+KeyLayout byte_layout()
+{
+    KeyLayout L;
+    for (int b = 0; b < 8; b++) {
+        L.shift[b] = (uint8_t)(8 * b);
+        L.bits[b] = 8;
+    }
+    return L;
+}
+
+// The scans' layout: bytes, except that r = 2K - 48 bits above bit 48 with
+// 8 < r < 16 go evenly to the passes of bytes 6 and 7 (those passes run only
+// on bits the scans sort globally, above the local pass's low bits).
+KeyLayout scan_layout(const mp_ctx *ctx)
+{
+    KeyLayout L = byte_layout();
+    const int r = 2 * ctx->cfg.k - 48;
+    int bits = ctx->cfg.sort_digit_bits;
+    if (const char *e = getenv("MP_SORT_DIGITS")) bits = atoi(e);  // tests run both layouts
+    if (bits != 8 && r > 8 && r < 16 && sort_local_bits(ctx->cfg.k) <= 48) {
+        const int w = (r + 1) / 2;
+        L.bits[6] = (uint8_t)w;
+        L.shift[7] = (uint8_t)(48 + w);
+        L.bits[7] = (uint8_t)(r - w);
+    }
+    return L;
+}
+
 uint32_t scan_pass_mask(int K)
 {
     const int S = sort_local_bits(K);
@@ -1552,8 +1592,9 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     // order except the quirk's first tuples and the tuple dump, which derive it
     const int S = sort_local_bits(ctx->cfg.k);
     const uint32_t mask = scan_pass_mask(ctx->cfg.k);
+    const KeyLayout KL = scan_layout(ctx);
     MP_TRY(radix_sort_generic(ctx, ctx->keys, ctx->lab, ctx->adj, &ctx->cur, n, mask, passes_out, nullptr, 0,
-                              nullptr, gen, segs));
+                              nullptr, gen, segs, &KL));
     if (S == 0 || n <= 1) return 0;
     const uint64_t wins = (n + kLWin - 1) / kLWin;
     // scratch: long-group count, moved-tuple total, long-group starts (one per window at most),

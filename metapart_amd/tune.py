@@ -6,9 +6,12 @@
   stays resident (FASTQ bytes, per-read state, packed streams, the writer's
   buffers), fits an HBM budget.
 * Sort tile: tune_sort_tile() times the radix pass with each tile shape the
-  library offers on a sample of the workload and returns the fastest.  Digits
-  are 8 bits: a 256-bin histogram per wave is what fits LDS next to a
-  6144-tuple staging tile at two tiles per CU.
+  library offers on a sample of the workload and returns the fastest.
+* Digit layout: tune_sort_digits() times the scans' sort with byte digits and
+  with the k-mer's top bits split over the last two passes (k = 29..31; at
+  k = 31 7 + 7 bits instead of 8 + 6) and returns the faster for
+  mp_cfg.sort_digit_bits.  Digits are at most 8 bits: a 256-bin histogram per
+  wave is what fits LDS next to a 6144-tuple staging tile at two tiles per CU.
 """
 from __future__ import annotations
 
@@ -63,12 +66,31 @@ def tune_sort_tile(k: int = 31, reads: int = 2_000_000, seed: int = 2, device: i
     times = {}
     for shape in TILE_SHAPES:
         with Context(k=k, device=device, sort_tile=shape) as ctx:
-            ctx.load(buf, [0, buf.nbytes])
             best = None
             for _ in range(2):  # the first run allocates
+                ctx.load(buf, [0, buf.nbytes])
                 r = partition(ctx, S=1, T=1, build_output=False)
                 ms = float(r.scans[0]["ms_sort_pass_avg"])
                 best = ms if best is None else min(best, ms)
             times[shape] = best
     buf.free()
     return {"best": min(times, key=times.get), "ms_per_pass": times}
+
+
+def tune_sort_digits(k: int = 31, reads: int = 2_000_000, seed: int = 2, device: int = 0) -> dict:
+    """Times the scans' sort (all passes and the local pass) of a sample
+    workload with byte digits (8) and with the split top (7); returns
+    {"best": sort_digit_bits, "ms_sort": {bits: ms}}."""
+    buf = synth(2, n_reads=reads, seed=seed)
+    times = {}
+    for bits in (8, 7):
+        with Context(k=k, device=device, sort_digit_bits=bits) as ctx:
+            best = None
+            for _ in range(2):  # the first run allocates
+                ctx.load(buf, [0, buf.nbytes])
+                r = partition(ctx, S=1, T=1, build_output=False)
+                ms = float(r.scans[0]["ms_sort"])
+                best = ms if best is None else min(best, ms)
+            times[bits] = best
+    buf.free()
+    return {"best": min(times, key=times.get), "ms_sort": times}

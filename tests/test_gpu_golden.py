@@ -32,3 +32,18 @@ def test_gpu_tools_match_reference_with_64bit_status(name, tmp_path, monkeypatch
     case = next(c for c in CASES if c["name"] == name)
     got = run_chain("gpu", case, str(tmp_path), timeout=300)
     assert diff_hashes(got, case["sha256"]) == []
+
+
+BYTE_DIGITS = ["c1_S3T2_k31", "small_k31_S2T2", "gcrep_k31_S4T2", "il_k31_T2", "mixed_k31"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", BYTE_DIGITS)
+def test_gpu_tools_match_reference_with_byte_digits(name, tmp_path, monkeypatch):
+    """The same with every radix pass on a whole byte of the k-mer (at k = 31
+    the default splits the top 14 bits over the last two passes, 7 + 7)."""
+    print("gpu golden case (byte digits)", name, flush=True)
+    monkeypatch.setenv("MP_SORT_DIGITS", "8")
+    case = next(c for c in CASES if c["name"] == name)
+    got = run_chain("gpu", case, str(tmp_path), timeout=300)
+    assert diff_hashes(got, case["sha256"]) == []
