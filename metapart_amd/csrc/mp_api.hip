@@ -78,6 +78,7 @@ int mp_ctx_destroy(mp_ctx *ctx)
         if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
         if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
     }
+    if (ctx->pair_host) (void)hipHostFree(ctx->pair_host);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

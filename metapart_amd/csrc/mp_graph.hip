@@ -1272,11 +1272,13 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
     uint32_t *gparent = (uint32_t *)ctx->gparent.p, *gsize = (uint32_t *)ctx->gsize.p;
     uint32_t *ordered = (uint32_t *)ctx->wg_cnt.p;
     uint64_t n_ordered = 0;
-    std::vector<uint64_t> hpairs;
-    std::vector<uint32_t> cs;
+    // host side, in the ctx's pinned buffer: the ordered pairs, the compact
+    // roots' sizes, the components' ranges of pairs (first [0, Kg), ends
+    // [Kg, 2 Kg)) and the compact roots' final roots
+    uint64_t *hpairs = nullptr;
+    uint32_t *cs = nullptr, *cfl = nullptr, *cp = nullptr;
     uint64_t M = 0;   // roots in candidate-graph components of > 1000 reads
     uint64_t Kg = 0;  // those components
-    std::vector<uint32_t> cfl;  // their ranges of ordered pairs: first [0, Kg), ends [Kg, 2 Kg)
     uint32_t *d_flag = nullptr, *d_csize = nullptr, *d_corig = nullptr, *d_cfinal = nullptr, *d_cfl = nullptr;
     uint64_t *d_cid = nullptr, *d_pairs = nullptr;
     if (P) {
@@ -1342,12 +1344,23 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
                                (const uint64_t *)ctx->items[cur].p, (const uint32_t *)ctx->ilens[cur].p,
                                (const uint64_t *)d_pairs, n_ordered, d_sorted, d_cfl, d_cfl + Kg);
             MP_HIP(hipGetLastError());
-            hpairs.resize(n_ordered);
-            cs.resize(M);
-            cfl.resize(2 * Kg);
-            MP_HIP(hipMemcpyAsync(hpairs.data(), d_sorted, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(cs.data(), d_csize, M * 4, hipMemcpyDeviceToHost, ctx->stream));
-            MP_HIP(hipMemcpyAsync(cfl.data(), d_cfl, 2 * Kg * 4, hipMemcpyDeviceToHost, ctx->stream));
+            const uint64_t hb = n_ordered * 8 + (2 * M + 2 * Kg) * 4 + 64;
+            if (ctx->pair_host_bytes < hb) {
+                MP_HIP(hipStreamSynchronize(ctx->stream));
+                if (ctx->pair_host) MP_HIP(hipHostFree(ctx->pair_host));
+                ctx->pair_host = nullptr;
+                ctx->pair_host_bytes = 0;
+                const uint64_t want = hb + hb / 4;  // room to grow
+                MP_HIP(hipHostMalloc(&ctx->pair_host, want, hipHostMallocDefault));
+                ctx->pair_host_bytes = want;
+            }
+            hpairs = (uint64_t *)ctx->pair_host;
+            cs = (uint32_t *)(hpairs + n_ordered);
+            cfl = cs + M;
+            cp = cfl + 2 * Kg;
+            MP_HIP(hipMemcpyAsync(hpairs, d_sorted, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(cs, d_csize, M * 4, hipMemcpyDeviceToHost, ctx->stream));
+            MP_HIP(hipMemcpyAsync(cfl, d_cfl, 2 * Kg * 4, hipMemcpyDeviceToHost, ctx->stream));
             MP_HIP(hipFreeAsync(d_cfl, ctx->stream));
         }
         hipLaunchKernelGGL(pair_apply_kernel, dim3(stride_grid(N)), dim3(256), 0, ctx->stream, parent,
@@ -1360,7 +1373,9 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         MP_HIP(hipStreamSynchronize(ctx->stream));
         const auto t0 = std::chrono::steady_clock::now();
         // parent and size side by side: a find ends on the line it reads the size from
-        std::vector<PairNode> nd(M);
+        static_assert(sizeof(PairNode) == sizeof(uint64_t), "one node per 64-bit word");
+        if (ctx->pair_nodes.size() < M) ctx->pair_nodes.resize(M);  // kept by the ctx: no page faults after the first call
+        PairNode *nd = (PairNode *)ctx->pair_nodes.data();
         for (uint64_t c = 0; c < M; c++) nd[c] = {(uint32_t)c, cs[c]};
         // each component's pairs in pair order, the reference's loop; the
         // components touch disjoint nodes
@@ -1382,8 +1397,7 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
                         __builtin_prefetch(&nd[nd[(uint32_t)f].p]);
                         __builtin_prefetch(&nd[nd[(uint32_t)(f >> 32)].p]);
                     }
-                    const uint32_t a = hfind(nd.data(), (uint32_t)hpairs[q]),
-                                   b = hfind(nd.data(), (uint32_t)(hpairs[q] >> 32));
+                    const uint32_t a = hfind(nd, (uint32_t)hpairs[q]), b = hfind(nd, (uint32_t)(hpairs[q] >> 32));
                     if (a == b) continue;
                     if (nd[a].s <= MP_PAIR_GATE || nd[b].s <= MP_PAIR_GATE) {
                         tun[t]++;
@@ -1403,9 +1417,8 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
         for (unsigned t = 0; t < nth; t++) unions += tun[t];
         host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         // the final root of every compact root, by pointer jumping on the device
-        std::vector<uint32_t> cp(M);
         for (uint64_t c = 0; c < M; c++) cp[c] = nd[c].p;
-        MP_HIP(hipMemcpyAsync(d_cfinal, cp.data(), M * 4, hipMemcpyHostToDevice, ctx->stream));
+        MP_HIP(hipMemcpyAsync(d_cfinal, cp, M * 4, hipMemcpyHostToDevice, ctx->stream));
         MP_TRY(compress_labels(ctx, d_cfinal, nullptr, M));
         hipLaunchKernelGGL(pair_merge_kernel, dim3(nblk(N)), dim3(256), 0, ctx->stream, parent,
                            (const uint32_t *)gparent, (const uint32_t *)gsize, (const uint64_t *)d_cid,

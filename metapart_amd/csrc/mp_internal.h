@@ -161,6 +161,12 @@ struct mp_ctx {
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_used[2] = {false, false};
+    // the ordered pair step's host side: its pair list, sizes, component ranges
+    // and final roots in pinned memory (copied at the link's rate, no page
+    // faults after the first call), and its union-find nodes
+    void *pair_host = nullptr;
+    uint64_t pair_host_bytes = 0;
+    std::vector<uint64_t> pair_nodes;
     std::vector<uint64_t> chunk_ofs;  // byte offset of each chunk (C+1)
     int cur = 0;               // buffer holding the last scan's sorted tuples
     uint64_t last_n = 0;       // tuples of the last scan
