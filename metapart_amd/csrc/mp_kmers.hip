@@ -102,12 +102,15 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
     __shared__ uint64_t s_out[kHDigits];
     __shared__ uint32_t s_bh[PHASE == 0 ? 8 : 1][kSortRadix];
     __shared__ uint64_t s_words[kHWords];  // the group's 2-bit words, then its 'N' words
-    __shared__ uint32_t s_gc[GC ? kGenCntTiles : 1][kSortRadix];
+    // digit d of a generating-pass tile in the 16-bit half d & 1 of word d / 2 (a
+    // tile holds at most 2^16 - 1 windows): half the LDS, more workgroups per CU
+    __shared__ uint32_t s_gc[GC ? kGenCntTiles : 1][kSortRadix / 2];
+    static_assert(kSortRadix % 2 == 0, "two digits per word");
     const int tid = threadIdx.x;
     if (PHASE == 0) {
         for (int j = tid; j < 8 * kSortRadix; j += kKBlock) (&s_bh[0][0])[j] = 0;
         if (GC)
-            for (int j = tid; j < kGenCntTiles * kSortRadix; j += kKBlock) (&s_gc[0][0])[j] = 0;
+            for (int j = tid; j < kGenCntTiles * kSortRadix / 2; j += kKBlock) (&s_gc[0][0])[j] = 0;
     }
     const uint32_t t = blockIdx.x;
     int c = 0, hi = H.C - 1;  // chunk: last c with tile0[c] <= t
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
                     gnext += H.gtile;
                 }
                 const uint32_t d = (uint32_t)(canon >> H.gshift) & 0xFF;
-                if (gt - g_lo < (uint64_t)kGenCntTiles) atomicAdd(&s_gc[gt - g_lo][d], 1u);
+                if (gt - g_lo < (uint64_t)kGenCntTiles) atomicAdd(&s_gc[gt - g_lo][d >> 1], 1u << (16 * (d & 1)));
                 else atomicAdd(&H.gcnt[gt * kSortRadix + d], 1u);
             }
             const uint32_t b = (uint32_t)(canon >> P.bshift);
@@ -215,7 +218,8 @@ __global__ __launch_bounds__(kKBlock) void hist_tile_kernel(KParams P, HParams H
             const uint64_t ng = min(g_hi - g_lo + 1, (uint64_t)kGenCntTiles);
             for (uint64_t j = tid; j < ng * kSortRadix; j += kKBlock) {
                 const uint64_t g = g_lo + j / kSortRadix;
-                const uint32_t v = s_gc[j / kSortRadix][j % kSortRadix];
+                const uint32_t dd = (uint32_t)(j % kSortRadix);
+                const uint32_t v = (s_gc[j / kSortRadix][dd >> 1] >> (16 * (dd & 1))) & 0xFFFFu;
                 const uint64_t gw0 = origin + (g - tbase) * H.gtile, gw1 = min(gw0 + H.gtile, cend);
                 const bool owned = gw0 >= w_lo && gw1 <= w_hi;
                 if (owned) H.gcnt[g * kSortRadix + (j % kSortRadix)] = v;
@@ -316,6 +320,8 @@ static int count_plain_partitioned(mp_ctx *ctx, KParams P, const uint64_t *chunk
     ctx->gcnt_seg = seg;
     if (H.bmask && !ctx->arena && seg) {
         const uint64_t gtile = sort_tile_items(ctx);
+        if (gtile >= (1u << 16)) return ctx->fail(MP_EINVAL, "radix tile of %llu windows: counts need 16 bits",
+                                                 (unsigned long long)gtile);
         uint64_t npos = 0;
         MP_HIP(hipMemcpyAsync(&npos, (const uint64_t *)ctx->pos_off.p + ctx->N, sizeof(uint64_t),
                               hipMemcpyDeviceToHost, ctx->stream));
