@@ -115,7 +115,6 @@ static int load_common(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *c
 // staging buffers: host threads fill one while the DMA engine drains the
 // other, so pageable memory is read once at memory speed and PCIe runs at the
 // pinned rate.
-constexpr uint64_t kStageBytes = 256ull << 20;
 constexpr int kStageThreads = 8;
 
 static void parallel_copy(char *dst, const char *src, uint64_t n)

@@ -1344,18 +1344,33 @@ int finish_components(mp_ctx *ctx, uint64_t *W, mp_finish_stats *st)
                                (const uint64_t *)ctx->items[cur].p, (const uint32_t *)ctx->ilens[cur].p,
                                (const uint64_t *)d_pairs, n_ordered, d_sorted, d_cfl, d_cfl + Kg);
             MP_HIP(hipGetLastError());
-            const uint64_t hb = n_ordered * 8 + (2 * M + 2 * Kg) * 4 + 64;
-            if (ctx->pair_host_bytes < hb) {
+            // pinned: the ctx's own buffer when it is large enough, else the FASTQ
+            // upload's two staging buffers when present (idle once the upload
+            // has left) and each part fits one, else a new buffer of the ctx
+            // (pinning costs about 0.25 ms per MB, once)
+            const uint64_t need_a = (n_ordered * 8 + 63) & ~63ull, need_b = (2 * M + 2 * Kg) * 4 + 64;
+            uint8_t *pa = nullptr, *pb = nullptr;
+            if (ctx->pair_host_bytes >= need_a + need_b) {
+                pa = (uint8_t *)ctx->pair_host;
+                pb = pa + need_a;
+            } else if (ctx->stage[0] && ctx->stage[1] && need_a <= kStageBytes && need_b <= kStageBytes) {
+                for (int b = 0; b < 2; b++)
+                    if (ctx->stage_used[b]) MP_HIP(hipEventSynchronize(ctx->stage_ev[b]));
+                pa = (uint8_t *)ctx->stage[0];
+                pb = (uint8_t *)ctx->stage[1];
+            } else {
                 MP_HIP(hipStreamSynchronize(ctx->stream));
                 if (ctx->pair_host) MP_HIP(hipHostFree(ctx->pair_host));
                 ctx->pair_host = nullptr;
                 ctx->pair_host_bytes = 0;
-                const uint64_t want = hb + hb / 4;  // room to grow
+                const uint64_t want = (need_a + need_b) + (need_a + need_b) / 4;  // room to grow
                 MP_HIP(hipHostMalloc(&ctx->pair_host, want, hipHostMallocDefault));
                 ctx->pair_host_bytes = want;
+                pa = (uint8_t *)ctx->pair_host;
+                pb = pa + need_a;
             }
-            hpairs = (uint64_t *)ctx->pair_host;
-            cs = (uint32_t *)(hpairs + n_ordered);
+            hpairs = (uint64_t *)pa;
+            cs = (uint32_t *)pb;
             cfl = cs + M;
             cp = cfl + 2 * Kg;
             MP_HIP(hipMemcpyAsync(hpairs, d_sorted, n_ordered * 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -16,6 +16,7 @@ namespace mp {
 constexpr int kWave = 64;
 constexpr uint32_t kBad = 1;  // bit of bad[]: label marked by the frequency filter
 constexpr int kSortRadix = 256;
+constexpr uint64_t kStageBytes = 256ull << 20;  // each of a ctx's two pinned host staging buffers
 constexpr int kSortPasses = 10;  // radix passes: adj, k-mer bytes 0..7, sample of the label
 
 // ---------------------------------------------------------------- device helpers
