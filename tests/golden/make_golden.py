@@ -72,6 +72,8 @@ CASES = [
     ("var_k19", "var", 4, 19, 1, 1, "se", ""),
     ("short_k21", "short", 4, 21, 1, 1, "se", ""),
     ("short_min3_k21", "short", 4, 21, 1, 1, "se", "min"),
+    ("small_k30_S2T2", "small150", 8, 30, 2, 2, "se", ""),   # k = 29, 30: the scans' split top digits
+    ("gcrep_k29", "gc_rep", 8, 29, 1, 1, "se", ""),
 ]
 
 
@@ -235,6 +237,7 @@ def main():
             json.dump(res, f, indent=1, sort_keys=True)
     finally:
         shutil.rmtree(work, ignore_errors=True)
+    main_viewidx()  # every case's viewIdx hash too
 
 
 if __name__ == "__main__":

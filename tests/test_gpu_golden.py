@@ -34,7 +34,7 @@ def test_gpu_tools_match_reference_with_64bit_status(name, tmp_path, monkeypatch
     assert diff_hashes(got, case["sha256"]) == []
 
 
-BYTE_DIGITS = ["c1_S3T2_k31", "small_k31_S2T2", "gcrep_k31_S4T2", "il_k31_T2", "mixed_k31"]
+BYTE_DIGITS = ["c1_S3T2_k31", "small_k31_S2T2", "gcrep_k31_S4T2", "il_k31_T2", "mixed_k31", "small_k30_S2T2"]
 
 
 @pytest.mark.gpu
