@@ -45,17 +45,6 @@ struct GParams {
     unsigned long long *stats;  // [0] evaluated runs, [1] zeroed runs
 };
 
-__device__ __forceinline__ void seg_of(const GParams &G, uint64_t i, uint64_t &s0, uint64_t &s1)
-{
-    int lo = 0, hi = G.T - 1;  // last t with seg[t] <= i
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (G.seg[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    s0 = G.seg[lo];
-    s1 = G.seg[lo + 1];
-}
-
 __global__ void seg_pos_kernel(const uint64_t *__restrict__ keys, uint64_t n, int bshift,
                                const uint64_t *__restrict__ bounds, int T, uint64_t *__restrict__ pos)
 {
@@ -153,15 +142,15 @@ __device__ int eval_run_mem(const GParams &G, uint64_t i, uint64_t j)
     return 1;
 }
 
-// Filter over tiles of 2048 tuples, looking 256 tuples past the tile so that
-// a run crossing its end is usually whole.  Nothing of the tuples is staged
-// in LDS but bitmaps: lane l of the workgroup owns tuples l, l+256, ... (a
-// wave owns 64 consecutive tuples, a "word").  Three steps:
-//   1. per word, wave ballots give the run starts (segment starts included),
-//      the tuples with a non-zero k-mer and three bit planes each of the in
-//      and out neighbour classes; a run of one tuple (the next tuple of the
-//      word starts a run too) has no verdict, which its lane records at once,
-//      and the tile's other run starts go to a list;
+// Filter over tiles of 6144 tuples, looking 256 tuples past the tile so that
+// a run crossing its end is usually whole.  Of the tuples only bitmaps are
+// staged in LDS: lane l of the workgroup owns tuples l, l+256, ... (a wave
+// owns 64 consecutive tuples, a "word").  Three steps:
+//   1. per word, wave ballots give the run starts (a key change, or a segment
+//      start from the tile's segment bitmap) and six bit planes of the adj
+//      byte (three each of the in and out neighbour classes); a run of one
+//      tuple (the next tuple of the word starts a run too) has no verdict,
+//      and the wave's other run starts go to the wave's list;
 //   2. one lane per listed run finds the run's end in the start bitmap, its
 //      class tallies by popcounts over the run's words, and its verdict; a run
 //      longer than the tuples looked at is evaluated from memory;
@@ -169,54 +158,125 @@ __device__ int eval_run_mem(const GParams &G, uint64_t i, uint64_t j)
 //      its label (the tile before owns the runs that start before).
 // Besides the filter's effects it records two bitmaps for the union: every
 // run start and the starts of the runs the union must join (k-mer not zero
-// and not zeroed, two tuples or more).
-constexpr int kFLanes = 256, kFPer = 8, kFTile = kFLanes * kFPer, kFHalo = 256, kFStage = kFTile + kFHalo,
-              kFIter = kFStage / kFLanes, kFWords = kFStage / 64, kFSeg = 64;
+// and not zeroed, two tuples or more).  Word positions, masks and list
+// counts are wave-uniform and stay in scalar registers, and no step searches
+// per tuple.  The tile's fixed cost (its barriers, the segment bitmap, the
+// label gather, the counters) is what bounds the kernel, so the tile is as
+// large as keeps the workgroups a CU holds (filter time at 20 M reads,
+// k = 31: 2048 tuples a tile 23.0 ms, 4096 12.5, 6144 10.9, 8192 11.8).
+constexpr int kFLanes = 256, kFPer = 24, kFTile = kFLanes * kFPer, kFHalo = 256, kFStage = kFTile + kFHalo,
+              kFIter = kFStage / kFLanes, kFWords = kFStage / 64, kFWaves = kFLanes / 64,
+              kFList = 33 * kFPer;  // a wave's listed runs: at most 33 a word (32 of two tuples or more, and the last)
+static_assert(kFWaves == 4, "the run lists assume 4 waves");
+static_assert(kFIter <= 64, "one bit a word in the marking step's mask");
 
-__device__ __forceinline__ uint32_t class_count(const uint64_t *pl, uint64_t m, int c)
+// the number of bits of m below this lane
+__device__ __forceinline__ uint32_t bits_below(uint64_t m)
 {
-    m &= (c & 1) ? pl[0] : ~pl[0];
-    m &= (c & 2) ? pl[1] : ~pl[1];
-    m &= (c & 4) ? pl[2] : ~pl[2];
-    return (uint32_t)__popcll(m);
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// tuples of class C (0-6) among the ones in a (the mask with plane 0's bit
+// of C applied): a & (plane 1 or its complement) & (plane 2 or its
+// complement), one three-input bit operation a 32-bit half (its truth table
+// holds the one row a = 1, p1 = bit 1 of C, p2 = bit 2 of C)
+template <int C>
+__device__ __forceinline__ uint32_t class_tally(uint64_t a, uint64_t p1, uint64_t p2)
+{
+    constexpr unsigned kRow = 4 | ((C & 2) ? 2 : 0) | ((C & 4) ? 1 : 0);
+    return (uint32_t)__builtin_popcount(__builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)p1, (uint32_t)p2, 1u << kRow)) +
+           (uint32_t)__builtin_popcount(__builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(p1 >> 32),
+                                                                    (uint32_t)(p2 >> 32), 1u << kRow));
+}
+
+// tallies of the tuples in m by class (0-6) from the in side's three bit
+// planes (class bit b in plane b) and the out side's (planes 3-5): class c's
+// in tally in the low half of io[c], its out tally in the high half (a run
+// here holds at most kFStage < 2^16 tuples; half the registers keep the run
+// loop from spilling)
+__device__ __forceinline__ void tally_classes(const uint64_t *pl, uint64_t m, uint32_t io[7])
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++) {  // one side at a time: fewer masks live
+        const uint64_t p0 = pl[3 * s], p1 = pl[3 * s + 1], p2 = pl[3 * s + 2];
+        const uint64_t a0 = m & ~p0, a1 = m & p0;
+        io[0] += class_tally<0>(a0, p1, p2) << (16 * s);
+        io[1] += class_tally<1>(a1, p1, p2) << (16 * s);
+        io[2] += class_tally<2>(a0, p1, p2) << (16 * s);
+        io[3] += class_tally<3>(a1, p1, p2) << (16 * s);
+        io[4] += class_tally<4>(a0, p1, p2) << (16 * s);
+        io[5] += class_tally<5>(a1, p1, p2) << (16 * s);
+        io[6] += class_tally<6>(a0, p1, p2) << (16 * s);
+    }
+}
+
+// run_verdict of a staged run (len <= kFStage) in the form step 3 applies:
+// bit 0 marks (kVMark), bit 1 the out side (kVOut; else kVIn), bit 2 every
+// tuple (kVRatio), bits 3-5 the side's largest class, which a tuple of the
+// side's other classes is filtered for; 0 = nothing to mark.  zero: both
+// sides branch (kVZero).  io: tally_classes' packed tallies.
+__device__ __forceinline__ uint32_t staged_verdict(const uint32_t io[7], uint32_t len, bool &zero)
+{
+    uint32_t nIn = 0, nOut = 0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+        nIn += (io[c] & 0xFFFF) != 0;
+        nOut += (io[c] >> 16) != 0;
+    }
+    zero = nIn > 1 && nOut > 1;
+    if (zero || (nIn <= 1 && nOut <= 1)) return 0;
+    // one side branches: the other's second-largest count is 0, so only this
+    // side decides the ratio.  The two largest of count << 3 | (7 - class):
+    // the largest count at its lowest class (two classes are present, so it
+    // is not an empty one), and the second-largest count with multiplicity.
+    const bool side_in = nIn > 1;
+    uint32_t b1 = 0, b2 = 0;
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+        const uint32_t v = (((io[c] >> (side_in ? 0 : 16)) & 0xFFFF) << 3) | (uint32_t)(7 - c);
+        b2 = max(b2, min(b1, v));
+        b1 = max(b1, v);
+    }
+    const uint32_t arg = 7 - (b1 & 7), second = b2 >> 3;
+    // (float)second / (float)len > 0.2 in double (metapart.c:2356-2359,
+    // 2419): the rounded quotient exceeds 0.2 exactly when it is at least
+    // 0x3E4CCCCD, i.e. when second / len lies above the midpoint
+    // 26843545 * 2^-27 below it (a tie rounds to the even 0x3E4CCCCC); both
+    // operands are below 2^24, so the float conversions are exact
+    const bool ratio = ((uint64_t)second << 27) > 26843545ull * len;
+    return 1u | (side_in ? 0u : 2u) | (ratio ? 4u : 0u) | (arg << 3);
 }
 
 __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint64_t *__restrict__ rs_bits,
                                                                  uint64_t *__restrict__ st_bits)
 {
-    __shared__ uint64_t s_st[kFWords], s_nz[kFWords], s_rs[kFWords];
-    __shared__ uint64_t s_pl[kFWords][6];  // bits 0-2 of the in class, then of the out class
-    __shared__ uint16_t s_vd[kFTile];      // verdicts at the tile's run starts
-    __shared__ uint16_t s_list[kFTile];    // the tile's run starts
-    __shared__ uint8_t s_adj[kFStage];     // the tuples' adj bytes
-    __shared__ uint64_t s_seg[kFSeg + 1];
-    __shared__ uint32_t s_n;
+    __shared__ uint64_t s_st[kFWords], s_rs[kFWords], s_sg[kFWords];  // run starts, union runs, segment starts
+    __shared__ uint64_t s_pl[kFWords][6];                 // bits 0-2 of the in class, then of the out class
+    __shared__ int32_t s_ls[kFWords];                     // the last run start up to each word's end, -1: none
+    __shared__ __align__(16) uint8_t s_vd[kFTile];        // verdicts (staged_verdict) at the listed run starts
+    __shared__ uint16_t s_list[kFWaves][kFList];          // each wave's listed run starts
+    __shared__ uint32_t s_nl[kFWaves], s_anyseg;
+    __shared__ unsigned long long s_tail;  // the first segment boundary at or past the tuples looked at
     const uint64_t t0 = (uint64_t)blockIdx.x * kFTile;
     const uint32_t cnt = (uint32_t)min((uint64_t)kFTile, G.n - t0);    // run starts owned
     const uint32_t cnt_s = (uint32_t)min((uint64_t)kFStage, G.n - t0);  // tuples looked at
-    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const bool seg_lds = G.T <= kFSeg;
-    if (seg_lds && threadIdx.x <= (unsigned)G.T) s_seg[threadIdx.x] = G.seg[threadIdx.x];
-    if (threadIdx.x < kFWords) s_rs[threadIdx.x] = 0;
-    if (threadIdx.x == 0) s_n = 0;
-    uint64_t s0 = 1, s1 = 0;  // no segment yet
-    auto seg = [&](uint64_t i) {  // s0, s1: the segment holding tuple i
-        if (i >= s0 && i < s1) return;
-        const uint64_t *sg = seg_lds ? s_seg : G.seg;
-        int lo = 0, hi = G.T - 1;  // last t with seg[t] <= i
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (sg[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        s0 = sg[lo];
-        s1 = sg[lo + 1];
-    };
-    uint32_t ev = 0, z = 0;  // runs evaluated (the last run of a segment is not), runs zeroed
-    // each iteration's key, previous key and adj byte are loaded one iteration ahead
+    const uint32_t tid = threadIdx.x, lane = tid % kWave;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    if (tid < kFWords) {
+        s_rs[tid] = 0;
+        s_sg[tid] = 0;
+    }
+    if (tid == 0) {
+        s_tail = ~0ull;
+        s_anyseg = 0;
+    }
+    static_assert(kFTile % 16 == 0, "s_vd cleared 16 bytes a store");
+    for (uint32_t x = tid; x < kFTile / 16; x += kFLanes) reinterpret_cast<uint4 *>(s_vd)[x] = make_uint4(0u, 0u, 0u, 0u);
+    // each word's key, previous key and adj byte are loaded one word ahead
     uint64_t nk = 0, nkp = 0;
     uint32_t na = 0;
     auto load = [&](int q) {
-        const uint32_t j = q * kFLanes + threadIdx.x;
+        const uint32_t j = q * kFLanes + tid;
         if (j < cnt_s) {
             const uint64_t i = t0 + j;
             nk = G.keys[i];
@@ -224,139 +284,172 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
             na = G.adj[i];
         }
     };
+    // the segment starts among the tuples looked at (each starts a run; the
+    // filter evaluates no run that ends its segment) and the first boundary
+    // past them.  The positions run from seg[0] = 0 to seg[T] = n; the first
+    // 256 are read with the first word's tuples, before the barrier.
+    const uint64_t sp0 = tid <= (uint32_t)G.T ? G.seg[tid] : ~0ull;
     load(0);
     __syncthreads();
-#pragma unroll 1
+    auto note_seg = [&](uint32_t t, uint64_t p) {
+        if (t < (uint32_t)G.T && p >= t0 && p < t0 + cnt_s) {
+            atomicOr((unsigned long long *)&s_sg[(p - t0) / kWave], 1ull << ((p - t0) % kWave));
+            s_anyseg = 1;
+        }
+        if (p >= t0 + cnt_s && p != ~0ull) atomicMin(&s_tail, (unsigned long long)p);
+    };
+    note_seg(tid, sp0);
+    for (uint32_t t = tid + kFLanes; t <= (uint32_t)G.T; t += kFLanes) note_seg(t, G.seg[t]);
+    __syncthreads();
+    const bool anyseg = s_anyseg != 0;
+    uint32_t ev = 0, z = 0;     // runs evaluated (the last run of a segment is not), runs zeroed
+    uint32_t nl = 0, evs = 0;   // the wave's listed runs, its evaluated runs of one tuple
+#pragma unroll
     for (int q = 0; q < kFIter; q++) {
-        const uint32_t j = q * kFLanes + threadIdx.x;
-        const bool valid = j < cnt_s;
+        const uint32_t jw = q * kFLanes + wave * kWave;  // the word's first tuple
         const uint64_t k = nk, kp = nkp;
         const uint32_t an = na;
         if (q + 1 < kFIter) load(q + 1);
-        bool st = false, nz = false;
-        uint32_t a = 0;
-        if (valid) {
-            const uint64_t i = t0 + j;
-            a = an;
-            seg(i);
-            st = i == s0 || kp != k;
-            nz = k != 0;
+        if (jw >= cnt_s) break;
+        const uint32_t j = jw + lane;
+        const bool valid = j < cnt_s;
+        const uint32_t a = valid ? an : 0u;
+        uint64_t sg = 0;  // uniform: read into scalar registers, so that the masks below are scalar work
+        if (anyseg) {
+            const uint64_t v = s_sg[jw / kWave];
+            sg = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+                 __builtin_amdgcn_readfirstlane((uint32_t)v);
         }
-        s_adj[j] = (uint8_t)a;  // for step 3 (in LDS: registers are what the run loop lacks)
-        const uint64_t mst = __ballot(st), mnz = __ballot(nz);
+        const uint64_t mst = __builtin_amdgcn_ballot_w64(valid && kp != k) | sg;
         uint64_t pl[6];
 #pragma unroll
         for (int b = 0; b < 6; b++) pl[b] = __builtin_amdgcn_ballot_w64((a >> b) & 1);  // a = 0 past the tuples
-        const uint32_t jw = q * kFLanes + wave * kWave;  // first tuple of this wave's word
         const uint64_t own = jw + 64 <= cnt ? ~0ull : (jw < cnt ? (1ull << (cnt - jw)) - 1 : 0);
         // runs of one tuple: the tuple after it (same word) starts a run.  Its
         // tallies hold one class a side, so no verdict; it is evaluated unless
-        // it ends its segment (it cannot end the tuples: the next one is valid)
+        // the next tuple starts a segment (it cannot end the tuples: the next
+        // one is valid)
         const uint64_t msingle = mst & (mst >> 1) & own, mlist = mst & own & ~msingle;
-        if ((msingle >> lane) & 1) {
-            s_vd[j] = 0;
-            ev += t0 + j + 1 != s1;
-        }
-        uint32_t base = 0;
-        if (lane == 0 && mlist) base = atomicAdd(&s_n, (uint32_t)__popcll(mlist));
-        base = __shfl(base, 0, kWave);
-        if ((mlist >> lane) & 1) s_list[base + (uint32_t)__popcll(mlist & lanemask_lt())] = (uint16_t)j;
-        if (lane == 0 && jw < cnt_s) {
+        evs += (uint32_t)__popcll(msingle & ~(sg >> 1));
+        if ((mlist >> lane) & 1) s_list[wave][nl + bits_below(mlist)] = (uint16_t)j;
+        nl += (uint32_t)__popcll(mlist);
+        if (lane == 0) {
             const uint32_t wd = jw / kWave;
             s_st[wd] = mst;
-            s_nz[wd] = mnz;
 #pragma unroll
             for (int b = 0; b < 6; b++) s_pl[wd][b] = pl[b];
             if (jw < cnt) st_bits[(t0 + jw) / 64] = mst;
         }
     }
+    if (lane == 0) s_nl[wave] = nl;
     __syncthreads();
-    const uint32_t nw = (cnt_s + kWave - 1) / kWave, nruns = s_n;
-    for (uint32_t r = threadIdx.x; r < nruns; r += kFLanes) {
-        const uint32_t j = s_list[r];
+    const uint32_t nw = (cnt_s + kWave - 1) / kWave;
+    if (wave == 0) {  // s_ls: a max-scan of each word's last start, 64 words a round
+        int carry = -1;
+        for (uint32_t w0 = 0; w0 < (uint32_t)kFWords; w0 += kWave) {
+            const uint32_t w = w0 + lane;
+            const uint64_t v = w < nw ? s_st[w] : 0;
+            int ls = v ? (int)(w * kWave + 63 - (uint32_t)__builtin_clzll(v)) : carry;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int o = __shfl_up(ls, d, kWave);
+                if (lane >= (uint32_t)d) ls = max(ls, o);
+            }
+            if (w < (uint32_t)kFWords) s_ls[w] = ls;
+            carry = __shfl(ls, kWave - 1, kWave);
+        }
+    }
+    const uint32_t n0 = s_nl[0], n1 = n0 + s_nl[1], n2 = n1 + s_nl[2], nruns = n2 + s_nl[3];
+    const uint64_t tail = s_tail;
+    for (uint32_t r = tid; r < nruns; r += kFLanes) {
+        const uint32_t lw = (r >= n0) + (r >= n1) + (r >= n2);
+        const uint32_t j = s_list[lw][r - (lw == 0 ? 0 : lw == 1 ? n0 : lw == 2 ? n1 : n2)];
         const uint64_t i = t0 + j;
-        seg(i);
+        // the run's k-mer is zero only if the run begins its segment (the
+        // segment's tuples are sorted, 0 is the smallest k-mer): read from
+        // memory then, or for a run that goes on past the tuples looked at
+        const bool seg_start = anyseg && ((s_sg[j / kWave] >> (j % kWave)) & 1);
         // the run ends at the next start among the tuples looked at, else it continues past them
         uint32_t w = j / kWave;
         uint64_t b = (j % kWave) == 63 ? 0 : s_st[w] & (~0ull << (j % kWave + 1));
         while (!b && ++w < nw) b = s_st[w];
         const uint32_t je = b ? w * kWave + (uint32_t)__builtin_ctzll(b) : cnt_s;
         uint64_t e = t0 + je;
-        if (je == cnt_s) {
+        bool evald;  // the last run of a segment is never evaluated
+        if (je < cnt_s) {
+            // its segment ends at the next segment start, which is no earlier than its end
+            evald = !(anyseg && ((s_sg[je / kWave] >> (je % kWave)) & 1));
+        } else {
+            // no run, so no segment, starts after it here: its segment ends at the first boundary past
             const uint64_t k = G.keys[i];
-            while (e < s1 && G.keys[e] == k) e++;
+            while (e < tail && G.keys[e] == k) e++;
+            evald = e != tail || (G.open_last && e == G.n);
         }
         bool zeroed = false;
-        uint32_t v = 0;
-        if (e != s1 || (G.open_last && e == G.n)) {  // the last run of a segment is never evaluated
+        uint32_t c = 0;
+        if (evald) {
             ev++;
             if (e <= t0 + cnt_s) {
-                // in and out tallies of class c in the halves of io[c] (a run here
-                // holds at most kFStage < 2^16 tuples): half the registers, which
-                // keeps the run loop from spilling
                 uint32_t io[7];
 #pragma unroll
-                for (int c = 0; c < 7; c++) io[c] = 0;
+                for (int x = 0; x < 7; x++) io[x] = 0;
                 for (uint32_t wd = j / kWave; wd * kWave < je; wd++) {
                     const uint32_t lo = max(j, wd * kWave) - wd * kWave, hi = min(je, (wd + 1) * kWave) - wd * kWave;
                     const uint64_t m = (hi == 64 ? ~0ull : ((1ull << hi) - 1)) & ~((1ull << lo) - 1);
-#pragma unroll
-                    for (int c = 0; c < 7; c++)
-                        io[c] += class_count(s_pl[wd], m, c) | (class_count(s_pl[wd] + 3, m, c) << 16);
+                    tally_classes(s_pl[wd], m, io);
                 }
-                uint32_t in[8], out[8];
-#pragma unroll
-                for (int c = 0; c < 7; c++) {
-                    in[c] = io[c] & 0xFFFF;
-                    out[c] = io[c] >> 16;
-                }
-                in[7] = out[7] = 0;
-                v = run_verdict(in, out, e - i);
-                if (v & kVZero) {
-                    zero_run(G, i, e);
-                    zeroed = true;
-                }
-            } else {  // longer than the tuples looked at: memory
+                c = staged_verdict(io, (uint32_t)(e - i), zeroed);
+                if (zeroed) zero_run(G, i, e);
+            } else {  // longer than the tuples looked at: memory (marks applied there)
                 zeroed = eval_run_mem(G, i, e) == 2;
             }
             z += zeroed;
         }
-        s_vd[j] = (uint16_t)((v & kVMark) ? (v & 0x1F) | (((v >> 8) & 7) << 5) | (((v >> 12) & 7) << 8) : 0);
-        if (((s_nz[j / kWave] >> (j % kWave)) & 1) && !zeroed && e - i >= 2)
+        s_vd[j] = (uint8_t)c;
+        if (!zeroed && e - i >= 2 && !(seg_start && G.keys[i] == 0))
             atomicOr((unsigned long long *)&s_rs[j / kWave], 1ull << (j % kWave));
     }
     __syncthreads();
-    if (threadIdx.x * 64 < cnt) rs_bits[t0 / 64 + threadIdx.x] = s_rs[threadIdx.x];
-    // the lane's filtered tuples first (bit q), then their labels loaded
-    // together, then the marks: one load latency instead of one per tuple
-    uint32_t fm = 0;
+    if (tid * 64 < cnt) rs_bits[t0 / 64 + tid] = s_rs[tid];
+    // each tuple's run start: the last start at or before it in its word, else
+    // the last one before the word (none: an earlier tile owns the run); its
+    // adj bits from the word's bit planes.  The lane's filtered tuples first
+    // (bit q), then their labels loaded 8 at a time, then their marks: one
+    // load latency for 8 tuples
+    const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
+    uint64_t fm = 0;
 #pragma unroll
     for (int q = 0; q < kFIter; q++) {
-        const uint32_t j = q * kFLanes + threadIdx.x;
-        if (j >= cnt_s) continue;
-        uint32_t w = j / kWave;
-        uint64_t b = s_st[w] & ((j % kWave) == 63 ? ~0ull : ((2ull << (j % kWave)) - 1));
-        while (!b && w > 0) b = s_st[--w];
-        if (!b) continue;  // the run began in an earlier tile, which evaluated it
-        const uint32_t sj = w * kWave + 63 - (uint32_t)__builtin_clzll(b);
-        if (sj >= cnt) continue;  // a run of the next tile
-        const uint32_t c = s_vd[sj];
-        if (!(c & kVMark)) continue;
-        const uint32_t v = (c & 0x1F) | (((c >> 5) & 7) << 8) | (((c >> 8) & 7) << 12);
-        if (tuple_filtered(v, s_adj[j])) fm |= 1u << q;
+        const uint32_t jw = q * kFLanes + wave * kWave;
+        if (jw >= cnt_s) break;
+        const uint32_t j = jw + lane;
+        uint32_t a = 0;
+#pragma unroll
+        for (int x = 0; x < 6; x++) a |= (uint32_t)((s_pl[jw / kWave][x] >> lane) & 1) << x;
+        const uint64_t b = s_st[jw / kWave] & upto;
+        const int prev = jw ? s_ls[jw / kWave - 1] : -1;
+        const int sj = b ? (int)(jw + 63 - (uint32_t)__builtin_clzll(b)) : prev;
+        const bool ok = j < cnt_s && sj >= 0 && sj < (int)cnt;
+        const uint32_t c = s_vd[ok ? sj : 0];  // no branch: the LDS reads issue together
+        const uint32_t f = (c & 1) & ((c >> 2) | (uint32_t)(((a >> (3 * ((c >> 1) & 1))) & 7) != (c >> 3)));
+        fm |= (uint64_t)(ok ? f : 0u) << q;
     }
-    uint32_t lv[kFIter];
+    constexpr int kFChunk = 8;
 #pragma unroll
-    for (int q = 0; q < kFIter; q++)
-        if ((fm >> q) & 1) lv[q] = G.lab[t0 + q * kFLanes + threadIdx.x];
-    // every load above completes here, once: the stores below then wait on nothing
-    // (left to itself the compiler waits before each store, the previous store included)
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged (gfx9 encoding)
+    for (int q0 = 0; q0 < kFIter; q0 += kFChunk) {
+        uint32_t lv[kFChunk];
 #pragma unroll
-    for (int q = 0; q < kFIter; q++)
-        if ((fm >> q) & 1) G.filtered[lv[q]] = 1;
-    block_add(ev, &G.stats[0]);
+        for (int u = 0; u < kFChunk; u++)
+            if (q0 + u < kFIter && ((fm >> (q0 + u)) & 1)) lv[u] = G.lab[t0 + (q0 + u) * kFLanes + tid];
+        // every load above completes here, once: the stores below then wait on nothing
+        // (left to itself the compiler waits before each store, the previous store included)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unchanged (gfx9 encoding)
+#pragma unroll
+        for (int u = 0; u < kFChunk; u++)
+            if (q0 + u < kFIter && ((fm >> (q0 + u)) & 1)) G.filtered[lv[u]] = 1;
+    }
+    block_add(ev + (lane == 0 ? evs : 0u), &G.stats[0]);
     block_add(z, &G.stats[1]);
 }
 
