@@ -612,6 +612,17 @@ static uint8_t *g_dump_tuples = NULL;
 static uint64_t g_dump_n = 0, g_dump_nreads = 0;
 static uint64_t *g_dump_parent = NULL;
 void orc_set_dump_scan(int scan) { g_dump_scan = scan; }
+/* per scan: runs the filter evaluated (each k-mer change inside a segment
+ * ends one, metapart.c:2288-2290) and runs it zeroed (both sides branch) */
+#define ORC_MAX_SCANS 256
+static uint64_t g_runs_eval[ORC_MAX_SCANS], g_runs_zero[ORC_MAX_SCANS];
+int orc_get_filter_counts(int scan, uint64_t *evaluated, uint64_t *zeroed)
+{
+    if (scan < 0 || scan >= ORC_MAX_SCANS) return -1;
+    *evaluated = g_runs_eval[scan];
+    *zeroed = g_runs_zero[scan];
+    return 0;
+}
 int orc_get_dump(uint8_t **tuples, uint64_t *n, uint64_t **parent_after, uint64_t *nreads)
 {
     *tuples = g_dump_tuples; *n = g_dump_n; *parent_after = g_dump_parent; *nreads = g_dump_nreads;
@@ -892,8 +903,9 @@ static int mp_sort(mp_t *m, int s, uint8_t *in, uint64_t n, const uint64_t *per_
 }
 
 /* markMischiefKmers (metapart.c:2270-2519) */
-static void mp_filter(mp_t *m, uint8_t *st, const uint64_t *lrc, const uint64_t *roff)
+static void mp_filter(mp_t *m, int scan, uint8_t *st, const uint64_t *lrc, const uint64_t *roff)
 {
+    uint64_t nev = 0, nz = 0;
     for (int seg = 0; seg < m->T; seg++) {
         uint64_t first = roff[seg];
         uint64_t curK = t_kmer(st + 14 * first), curOfs = first;
@@ -915,6 +927,8 @@ static void mp_filter(mp_t *m, uint8_t *st, const uint64_t *lrc, const uint64_t 
                     }
                 }
                 before[7] = after[7] = 0;
+                nev++;
+                nz += nIn > 1 && nOut > 1;
                 if (nIn > 1 || nOut > 1) {
                     float sInR = (float)sInC / deg; /* fp32 divide, compared as double (2423) */
                     float sOutR = (float)sOutC / deg;
@@ -942,6 +956,10 @@ static void mp_filter(mp_t *m, uint8_t *st, const uint64_t *lrc, const uint64_t 
             after[(a >> 3) & 7]++;
             deg++;
         }
+    }
+    if (scan >= 0 && scan < ORC_MAX_SCANS) {
+        g_runs_eval[scan] = nev;
+        g_runs_zero[scan] = nz;
     }
     /* 2500-2517 */
     for (uint64_t i = 0; i < m->N; i++)
@@ -1080,6 +1098,8 @@ int orc_metapart(const char *out_prefix, uint64_t buf_size, int K, int S, int T,
     (void)nIL;
     mp_t mm, *m = &mm;
     memset(m, 0, sizeof(*m));
+    memset(g_runs_eval, 0, sizeof(g_runs_eval));
+    memset(g_runs_zero, 0, sizeof(g_runs_zero));
     if (K <= 10 || K > 32) return fail(-1, "K must be in [11, 32]");
     if (S <= 0 || T <= 0 || T > 256) return fail(-1, "bad S/T");
     if (nPE > 0) return fail(-1, "two-file paired-end input is not supported");
@@ -1145,7 +1165,7 @@ int orc_metapart(const char *out_prefix, uint64_t buf_size, int K, int S, int T,
         rc = mp_sort(m, s, tuples, n, per_thread, &sorted, lrc, roff);
         free(tuples);
         if (rc) break;
-        mp_filter(m, sorted, lrc, roff);
+        mp_filter(m, s, sorted, lrc, roff);
         rc = mp_union(m, sorted, n, lrc, roff);
         if (!rc) mp_compress(m);
         if (s == g_dump_scan) {

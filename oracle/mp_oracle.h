@@ -74,6 +74,9 @@ int orc_plan_ranges(const uint64_t *hist, int S, int T, uint64_t *scan, uint64_t
  * (14-byte reference layout), set by orc_set_dump_scan before orc_metapart. */
 void orc_set_dump_scan(int scan);
 int orc_get_dump(uint8_t **tuples, uint64_t *n, uint64_t **parent_after, uint64_t *nreads);
+/* Runs the filter of scan `scan` of the last orc_metapart evaluated (one per
+ * k-mer change inside a segment) and zeroed (both neighbour sides branch). */
+int orc_get_filter_counts(int scan, uint64_t *evaluated, uint64_t *zeroed);
 
 /* last error message */
 const char *orc_last_error(void);

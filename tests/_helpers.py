@@ -138,6 +138,7 @@ def oracle():
                                     ct.c_void_p, ct.c_int]
         L.orc_plan_ranges.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p]
         L.orc_set_dump_scan.argtypes = [ct.c_int]
+        L.orc_get_filter_counts.argtypes = [ct.c_int, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_uint64)]
         L.orc_get_dump.argtypes = [ct.POINTER(ct.POINTER(ct.c_uint8)), ct.POINTER(ct.c_uint64),
                                    ct.POINTER(ct.POINTER(ct.c_uint64)), ct.POINTER(ct.c_uint64)]
         L.orc_last_error.restype = ct.c_char_p
@@ -152,9 +153,10 @@ def _argv(files):
 
 
 def oracle_run(fq_path, C, K, S, T, interleaved=False, min3=False, dump_scan=-1, out_dir=None):
-    """Oracle pipeline on one file.  Returns dict with final parent (N), and,
-    if dump_scan >= 0, the sorted filtered tuples (n, 14) and the parent after
-    that scan (BAD reads as 0xBADBADBAD)."""
+    """Oracle pipeline on one file.  Returns dict with final parent (N), the
+    filter's evaluated and zeroed run counts per scan, and, if dump_scan >= 0,
+    the sorted filtered tuples (n, 14) and the parent after that scan (BAD
+    reads as 0xBADBADBAD)."""
     L = oracle()
     a = (0, 0, 1) if interleaved else (0, 1, 0)
     files = _argv([fq_path])
@@ -172,6 +174,12 @@ def oracle_run(fq_path, C, K, S, T, interleaved=False, min3=False, dump_scan=-1,
     assert rc == 0, L.orc_last_error()
     res = {"parent": np.ctypeslib.as_array(par, shape=(nr.value,)).copy()}
     L.orc_free(par)
+    # per scan: (runs the filter evaluated, runs it zeroed)
+    res["filter_counts"] = []
+    for sc in range(S):
+        ev, zr = ct.c_uint64(0), ct.c_uint64(0)
+        assert L.orc_get_filter_counts(sc, ct.byref(ev), ct.byref(zr)) == 0
+        res["filter_counts"].append((ev.value, zr.value))
     if dump_scan >= 0:
         tp = ct.POINTER(ct.c_uint8)()
         n = ct.c_uint64(0)

@@ -1,7 +1,8 @@
 """Stage-by-stage parity of the HIP path (through the C ABI) with the oracle on
-the same seeded inputs: per-chunk m-mer counts, the sorted and filtered tuple
-array of a scan (reference 14-byte layout), BAD flags and parent labels after
-that scan, and the final parent labels after the pair step."""
+the same seeded inputs: per-chunk m-mer counts, the filter's evaluated and
+zeroed run counts of every scan, the sorted and filtered tuple array of a scan
+(reference 14-byte layout), BAD flags and parent labels after that scan, and
+the final parent labels after the pair step."""
 import os
 
 import numpy as np
@@ -28,6 +29,10 @@ CASES = [
     # chains past 1000 reads, so part of it runs in the reference's order
     ("pairs_ordered", dict(n_reads=20000, n_genomes=40, genome_len_min=4000, genome_len_max=8000, abund_sigma=1.0,
                            seed=8), 4, 31, 1, 1, False, False, 0),
+    # many thread segments: segment starts inside the filter's tiles and words,
+    # where a run of one tuple ends its segment and a run's end is a segment's
+    ("plain_k31_T16", dict(n_reads=12000, seed=49, read_len=150), 16, 31, 1, 16, False, False, 0),
+    ("plain_k25_S2T7", dict(n_reads=10000, seed=50), 14, 25, 2, 7, False, False, 1),
 ]
 ORDERED = {"pairs_ordered"}
 CASES.append(
@@ -79,6 +84,7 @@ def _check_stages(case, tmp_path):
         for s in range(S):
             lo, hi = int(scan[s]), int(scan[s + 1])
             st = ctx.scan(s, lo, hi, thread[s * T:(s + 1) * T + 1], int(hist[min(hi, mp.NBUCKETS)] - hist[lo]))
+            assert (st.evaluated_runs, st.zeroed_runs) == ref["filter_counts"][s]
             if name in LONG_GROUPS:
                 assert st.sort_long_groups > 0
             if s == dump:
