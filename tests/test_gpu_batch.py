@@ -1,11 +1,14 @@
 """Batched form: a batch of independent samples partitioned in one launch
 sequence gives, for every sample, the labels, plan and output files of a plain
-run of that sample alone (one chunk, one scan, one segment)."""
+run of that sample alone (one chunk, one scan, one segment), which equal the
+oracle's on the same sample."""
 import hashlib
 import os
 
 import numpy as np
 import pytest
+
+from _helpers import oracle_samples
 
 import metapart_amd as mp
 
@@ -42,19 +45,36 @@ def _solo(buf, out_dir):
     return r
 
 
+@pytest.fixture(scope="module")
+def oracle_ref(tmp_path_factory):
+    """The oracle on every sample alone: (labels, {output file: sha256})."""
+    d = tmp_path_factory.mktemp("oracle")
+    jobs = []
+    for i, b in enumerate(_samples()):
+        sd = d / str(i)
+        sd.mkdir()
+        (sd / "in.fastq").write_bytes(b.tobytes())
+        jobs.append((str(sd / "in.fastq"), 1, K, 1, 1, str(sd)))
+    return oracle_samples(jobs)
+
+
 # segmented: every sample's tuples sorted in tiles of their own, from its own
 # position (the default); sample_pass: look-back passes over all samples, then
 # a last pass on the sample of the label (MP_SORT_LOOKBACK=1)
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["segmented", "sample_pass"])
-def test_batch_equals_solo_runs(mode, tmp_path, monkeypatch):
-    print("gpu batch: 6 samples vs solo runs, %s sort" % mode, flush=True)
+def test_batch_equals_solo_runs(mode, tmp_path, monkeypatch, oracle_ref):
+    print("gpu batch: 6 samples vs solo runs and the oracle, %s sort" % mode, flush=True)
     if mode == "sample_pass":
         monkeypatch.setenv("MP_SORT_LOOKBACK", "1")
     bufs = _samples()
     solo = []
     for i, b in enumerate(bufs):
         solo.append(_solo(b, str(tmp_path / ("solo%d" % i))))
+        par, files = oracle_ref[i]
+        assert np.array_equal(solo[-1].parent.astype(np.uint32), par), i
+        rank0 = tmp_path / ("solo%d" % i) / "Rank0"
+        assert {f: hashlib.sha256((rank0 / f).read_bytes()).hexdigest() for f in sorted(os.listdir(rank0))} == files, i
     with mp.Context(k=K) as ctx:
         ctx.batch_load(bufs)
         res, st, fs = ctx.batch_partition()
