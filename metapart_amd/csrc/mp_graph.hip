@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
     if (lane == 0) s_nl[wave] = nl;
     __syncthreads();
     const uint32_t nw = (cnt_s + kWave - 1) / kWave;
-    if (wave == 0) {  // s_ls: a max-scan of each word's last start, 64 words a round
+    if (wave == 0) {  // s_ls: a max-scan of each word's last start, 64 words at a time
         int carry = -1;
         for (uint32_t w0 = 0; w0 < (uint32_t)kFWords; w0 += kWave) {
             const uint32_t w = w0 + lane;
