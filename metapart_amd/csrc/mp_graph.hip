@@ -316,9 +316,12 @@ __global__ __launch_bounds__(kFLanes, 8) void filter_tile_kernel(GParams G, uint
         const uint32_t a = valid ? an : 0u;
         uint64_t sg = 0;  // uniform: read into scalar registers, so that the masks below are scalar work
         if (anyseg) {
+            // readfirstlane returns a signed int: each half goes through uint32_t,
+            // or bit 31 of the low half would be copied into bits 32-63
             const uint64_t v = s_sg[jw / kWave];
-            sg = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-                 __builtin_amdgcn_readfirstlane((uint32_t)v);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)),
+                           lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+            sg = ((uint64_t)hi << 32) | lo;
         }
         const uint64_t mst = __builtin_amdgcn_ballot_w64(valid && kp != k) | sg;
         uint64_t pl[6];

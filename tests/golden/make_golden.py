@@ -74,6 +74,10 @@ CASES = [
     ("short_min3_k21", "short", 4, 21, 1, 1, "se", "min"),
     ("small_k30_S2T2", "small150", 8, 30, 2, 2, "se", ""),   # k = 29, 30: the scans' split top digits
     ("gcrep_k29", "gc_rep", 8, 29, 1, 1, "se", ""),
+    ("gcrep_k31_T16", "gc_rep", 16, 31, 1, 16, "se", ""),   # many thread segments a scan
+    # scan 0's fifth segment starts at bit 31 of a filter word (the filter's
+    # segment mask, read through signed 32-bit halves, once took bits 32-63 too)
+    ("small_k25_S2T8", "small150", 16, 25, 2, 8, "se", ""),
 ]
 
 
