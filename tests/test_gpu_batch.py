@@ -98,6 +98,40 @@ def test_batch_equals_solo_runs(mode, tmp_path, monkeypatch, oracle_ref):
 
 
 @pytest.mark.gpu
+def test_batch_of_many_small_samples_equals_solo_runs():
+    """150 samples of 40-400 reads of 60-150 bases: every sample is a segment
+    of the batch's filter and union, so the sample boundaries put segment
+    starts at nearly every bit position of the filter's 64-tuple words (a
+    solo run has none inside); labels and plans equal the solo runs'."""
+    print("gpu batch: 150 small samples vs solo runs", flush=True)
+    rng = np.random.default_rng(9)
+    bufs = [mp.synth(0, seed=900 + i, n_reads=2 * int(rng.integers(20, 201)), n_genomes=int(rng.integers(1, 4)),
+                     genome_len_min=3000, genome_len_max=6000, read_len_min=60, read_len_max=150, err_rate=0.01)
+            for i in range(150)]
+    solo_w, solo_par, ntup = [], [], []
+    with mp.Context(k=K) as ctx:
+        for b in bufs:
+            ctx.load(b, [0, b.nbytes])
+            r = mp.partition(ctx, S=1, T=1, out_prefix=None, want_parent=True)
+            solo_w.append(r.W.copy())
+            solo_par.append(r.parent.astype(np.uint32))
+            ntup.append(int(r.scans[0]["tuples"]))
+    # the batch's segment starts: the running tuple count (the samples in order)
+    starts = np.cumsum([0] + ntup[:-1]) % 64
+    print("  bit positions of the segment starts: %d of 64" % len(set(starts.tolist())), flush=True)
+    assert len(set(starts.tolist())) >= 48
+    with mp.Context(k=K) as ctx:
+        ctx.batch_load(bufs)
+        res, st, _ = ctx.batch_partition()
+        assert st.tuples == sum(ntup)
+        for i in range(len(bufs)):
+            assert np.array_equal(res[i]["W"], solo_w[i]), i
+            assert np.array_equal(ctx.batch_parent(i), solo_par[i]), i
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.gpu
 def test_batch_from_device_memory_and_single_sample(tmp_path):
     print("gpu batch: device-resident input", flush=True)
     import torch
