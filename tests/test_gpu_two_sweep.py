@@ -18,6 +18,8 @@ CASES = [
     ("plain_k25_S2_T3", dict(n_reads=24000, seed=62, read_len_min=50, read_len_max=160, n_rate=0.002), 6, 25, 2, 3,
      False, False),
     ("il_k31_T2", dict(n_reads=20000, seed=64, read_len=150, interleaved=1), 4, 31, 1, 2, False, True),
+    # eight thread segments: the parts begin and end inside segments
+    ("plain_k27_T8", dict(n_reads=24000, seed=65, read_len_min=60, read_len_max=150), 8, 27, 1, 8, False, False),
 ]
 
 
