@@ -1,7 +1,8 @@
 """Stage-by-stage parity with the oracle over 24 configurations drawn from a
 fixed seed: k = 13..32, one to three scans, one to eight thread segments
 (dividing the chunk count), plain and 3-minimizer, single-end and
-interleaved, read lengths from 15 to 200 bases, 'N' rates up to 0.5 %.  The
+interleaved, read lengths from 15 to 200 bases, 'N' rates up to 0.5 %, and in a
+quarter of them 5 % of the bases lower-case or other IUPAC letters.  The
 fixed cases elsewhere pin chosen corners; these put segment starts, run
 boundaries and tile ends at positions nobody chose.  The 3-minimizer variant
 and k = 32 take one scan (the reference's plans for more are degenerate)."""
@@ -30,8 +31,11 @@ def _configs(n=24, seed=2027):
                     n_genomes=int(rng.integers(2, 20)), genome_len_min=3000, genome_len_max=20000)
         if il:
             over["interleaved"] = 1
+        if rng.random() < 0.25:  # lower-case and other IUPAC letters in 5 % of the bases
+            over["mix"] = 0.05
         dump = int(rng.integers(0, S))
-        name = "r%02d_k%d_S%dT%d%s%s" % (i, K, S, T, "_min3" if min3 else "", "_il" if il else "")
+        name = "r%02d_k%d_S%dT%d%s%s%s" % (i, K, S, T, "_min3" if min3 else "", "_il" if il else "",
+                                          "_mix" if "mix" in over else "")
         out.append((name, over, C, K, S, T, min3, il, dump))
     return out
 

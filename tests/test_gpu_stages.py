@@ -66,9 +66,35 @@ def test_stages_match_oracle_with_64bit_status(name, tmp_path, monkeypatch):
     _check_stages(next(c for c in CASES if c[0] == name), tmp_path)
 
 
+def _mix_letters(buf, rate, seed):
+    """The FASTQ bytes with a share `rate` of the sequence bases changed: nine
+    in ten to lower case, the rest to another IUPAC letter or 'n' (bytes whose
+    2-bit code, adj byte and 'N' test the reference takes from the raw
+    character)."""
+    a = np.frombuffer(buf.tobytes(), dtype=np.uint8).copy()
+    nl = np.flatnonzero(a == ord("\n"))
+    starts, ends = nl[0::4] + 1, nl[1::4]  # sequence lines: after the 1st newline of a record, to the 2nd
+    inseq = np.zeros(a.size + 1, dtype=np.int32)
+    np.add.at(inseq, starts, 1)
+    np.add.at(inseq, ends, -1)
+    inseq = np.cumsum(inseq[:-1]) > 0
+    rng = np.random.default_rng(seed)
+    pick = inseq & (rng.random(a.size) < rate)
+    low = pick & (rng.random(a.size) < 0.9)
+    a[low] |= 0x20
+    other = np.frombuffer(b"RYKMSWBDHVn", dtype=np.uint8)
+    odd = pick & ~low
+    a[odd] = other[rng.integers(0, other.size, int(odd.sum()))]
+    return a
+
+
 def _check_stages(case, tmp_path):
     name, over, C, K, S, T, min3, il, dump = case
+    over = dict(over)
+    mix = over.pop("mix", 0.0)
     buf = mp.synth(0, **over)
+    if mix:
+        buf = _mix_letters(buf, mix, over.get("seed", 0))
     p = os.path.join(str(tmp_path), "in.fastq")
     open(p, "wb").write(buf.tobytes())
     ref = oracle_run(p, C, K, S, T, interleaved=il, min3=min3, dump_scan=dump)
