@@ -563,8 +563,8 @@ int mp_batch_partition(mp_ctx *ctx, mp_sample_result *res, mp_scan_stats *scan_s
     // from its own position) unless the passes take look-back offsets
     const char *lbe = getenv("MP_SORT_LOOKBACK");
     const bool segmented = !ctx->cfg.min3 && !(lbe && atoi(lbe) == 1);
-    if (ctx->cfg.min3) {
-        MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n));
+    if (ctx->cfg.min3) {  // every sample over its own one-scan bucket range
+        MP_TRY(enumerate_scan(ctx, 0, 0, MP_NBUCKETS, &n, true));
     } else {
         gen.reset(new GenSource());
         if (segmented) MP_TRY(batch_counts(ctx));

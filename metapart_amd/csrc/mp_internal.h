@@ -242,7 +242,7 @@ int batch_counts(mp_ctx *ctx);
 // per-bucket sums over chunks (u64 per bucket) of a C x (2^20+2) count table
 int bucket_totals(mp_ctx *ctx, const uint32_t *d_counts, int C, uint64_t *d_tot);
 // 3-minimizer k-mers of a scan: ordered enumeration into the tuple buffer 0
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out);
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool per_chunk_hi = false);
 // The scan's tuples generated by the first radix pass instead of an
 // enumeration: the reads' k-mers in [lo, hi) and their digit histograms.
 struct GenSource;

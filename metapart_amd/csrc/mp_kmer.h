@@ -34,9 +34,12 @@ struct KParams {
     const uint8_t *bad;
     const uint32_t *labels;  // when set: the labels of a scan taken at its start (a two-sweep scan)
     int scan0;
+    // when set (a batch, 3-minimizer): each chunk's own upper bucket bound,
+    // the one scan of that sample alone would have (chunk of read r from chunk_ids)
+    const uint32_t *chunk_hi;
 };
 
-enum { kModeHist = 0, kModeCount = 1, kModeEmit = 2 };
+enum { kModeHist = 0, kModeCount = 1, kModeEmit = 2, kModeTop = 3 };
 
 __device__ __forceinline__ uint8_t rc_char(uint8_t c)
 {

@@ -542,19 +542,32 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
         c = fwd ? f : rk.R;
         return true;
     };
+    // the chunk of read r (chunk_ids: C + 1 first read ids)
+    auto chunk_of = [&](uint64_t rr) {
+        int lo = 0, hi = P.C - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (P.chunk_ids[mid] <= rr) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
     if (MODE == kModeHist) {
         if (r < P.N) {
-            int lo = 0, hi = P.C - 1;
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (P.chunk_ids[mid] <= r) lo = mid; else hi = mid - 1;
-            }
-            uint32_t *cnts = P.counts + (size_t)lo * (MP_NBUCKETS + 2);
+            uint32_t *cnts = P.counts + (size_t)chunk_of(r) * (MP_NBUCKETS + 2);
             min3_count_read(L, P.K, P.bshift, kmer, [&](uint64_t b) { atomicAdd(&cnts[b], 1u); });
         }
         return;
     }
-    if (r < P.N) min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, kmer, [&](uint64_t, uint8_t) { cnt++; });
+    if (MODE == kModeTop) {  // counts[c]: 1 + the highest bucket of chunk c's minimizers
+        if (r < P.N) {
+            uint32_t *top = P.counts + chunk_of(r);
+            min3_count_read(L, P.K, P.bshift, kmer, [&](uint64_t b) { atomicMax(top, (uint32_t)b + 1u); });
+        }
+        return;
+    }
+    // the scan's bucket range, or the read's chunk's own (a batch of samples)
+    const uint64_t rhi = P.chunk_hi && r < P.N ? P.chunk_hi[chunk_of(r)] : P.hi;
+    if (r < P.N) min3_enum_read(s, L, P.K, P.bshift, P.lo, rhi, kmer, [&](uint64_t, uint8_t) { cnt++; });
     // exclusive scan of cnt over the workgroup
     uint32_t inc = cnt;
     for (int d = 1; d < kWave; d <<= 1) {
@@ -576,7 +589,7 @@ __global__ __launch_bounds__(kKBlock) void kmer_min3_kernel(KParams P)
     if (r >= P.N || cnt == 0) return;
     uint64_t pos = P.wg_off[blockIdx.x] + woff + inc - cnt;
     const uint32_t label = label_of(P, r);
-    min3_enum_read(s, L, P.K, P.bshift, P.lo, P.hi, kmer, [&](uint64_t k, uint8_t a) {
+    min3_enum_read(s, L, P.K, P.bshift, P.lo, rhi, kmer, [&](uint64_t k, uint8_t a) {
         P.keys[pos] = k;
         P.adj[pos] = a;
         P.lab[pos] = label;
@@ -842,12 +855,34 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
     return 0;
 }
 
-int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out)
+int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool per_chunk_hi)
 {
     if (!ctx->cfg.min3) return ctx->fail(MP_EINVAL, "k-mer tuples of every window come from the generating sort pass");
     KParams P = base_params(ctx);
     P.lo = lo;
     P.hi = hi;
+    uint64_t *d_ids = nullptr;
+    uint32_t *d_top = nullptr;
+    if (per_chunk_hi) {
+        // A batch: the window logic of enumerateKmers_3minimizers.c skips the
+        // k-mers outside the scan's bucket range, so every sample takes the
+        // range of its own one-scan plan, [0, 1 + its highest minimizer
+        // bucket) (mp_plan_ranges over its histogram), not the whole table
+        const int C = ctx->C;
+        MP_TRY(tmp_alloc(ctx, (void **)&d_ids, sizeof(uint64_t) * (C + 1) + sizeof(uint32_t) * (C + 1)));
+        d_top = (uint32_t *)(d_ids + (C + 1));
+        MP_HIP(hipMemcpyAsync(d_ids, ctx->chunk_ids.data(), sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+        MP_TRY(fill_u32(ctx, d_top, 0, C + 1));
+        P.chunk_ids = d_ids;
+        P.C = C;
+        P.counts = d_top;
+        hipLaunchKernelGGL(kmer_min3_kernel<kModeTop>, dim3((unsigned)((ctx->N + kKBlock - 1) / kKBlock)),
+                           dim3(kKBlock), 0, ctx->stream, P);
+        MP_HIP(hipGetLastError());
+        P.counts = nullptr;
+        P.chunk_hi = d_top;
+    }
     P.scan0 = s == 0;
     P.parent = (const uint32_t *)ctx->parent.p;
     P.bad = (const uint8_t *)ctx->bad.p;
@@ -875,6 +910,7 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
         hipLaunchKernelGGL(kmer_min3_kernel<kModeEmit>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
         MP_HIP(hipGetLastError());
     }
+    if (d_ids) MP_TRY(tmp_free(ctx, d_ids));
     ctx->cur = 0;
     *n_out = n;
     return 0;
