@@ -45,6 +45,16 @@ SYNTH_INPUTS = {
     # reads of 10-45 bases, 2 % 'N': many reads at or below k, and windows cut by 'N'
     "short": ["--preset", "0", "--reads", "30000", "--genomes", "4", "--len-min", "10", "--len-max", "45",
               "--err", "0.01", "--nrate", "0.02", "--seed", "9"],
+    # interleaved pairs of 60-160 bases with 3 % mixed letters and 0.5 % 'N'
+    "mixil": ["--preset", "0", "--reads", "16000", "--genomes", "5", "--len-min", "60", "--len-max", "160",
+              "--err", "0.01", "--nrate", "0.005", "--seed", "31", "--mix", "0.03", "--interleaved"],
+    # one 3 kb genome at ~1500x: k-mer groups of thousands of tuples
+    "deep": ["--preset", "0", "--reads", "30000", "--genomes", "1", "--glen-min", "3000", "--glen-max", "3000",
+             "--len", "150", "--err", "0.005", "--seed", "12"],
+    # 60 genomes of 2-6 kb, log-normal(1.0) abundances, 2 % errors: many small
+    # components and singleton k-mers
+    "many": ["--preset", "0", "--reads", "20000", "--genomes", "60", "--glen-min", "2000", "--glen-max", "6000",
+             "--sigma", "1.0", "--len", "100", "--err", "0.02", "--seed", "33"],
 }
 CASES = [
     ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
@@ -78,6 +88,12 @@ CASES = [
     # scan 0's fifth segment starts at bit 31 of a filter word (the filter's
     # segment mask, read through signed 32-bit halves, once took bits 32-63 too)
     ("small_k25_S2T8", "small150", 16, 25, 2, 8, "se", ""),
+    ("mixil_k23_S2T3", "mixil", 6, 23, 2, 3, "il", ""),
+    ("mixil_min3_k25", "mixil", 4, 25, 1, 1, "il", "min"),
+    ("deep_k31", "deep", 4, 31, 1, 1, "se", ""),
+    ("deep_k16_S2T2", "deep", 4, 16, 2, 2, "se", ""),   # k = 16: even, every sort pass global
+    ("many_k28_T4", "many", 8, 28, 1, 4, "se", ""),
+    ("many_min3_k31_T2", "many", 8, 31, 1, 2, "se", "min"),
 ]
 
 
