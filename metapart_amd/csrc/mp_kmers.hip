@@ -855,6 +855,8 @@ int gen_histograms(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint32_t pass_m
     return 0;
 }
 
+static int enumerate_tuples(mp_ctx *ctx, int s, KParams &P, uint64_t *n_out);
+
 int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out, bool per_chunk_hi)
 {
     if (!ctx->cfg.min3) return ctx->fail(MP_EINVAL, "k-mer tuples of every window come from the generating sort pass");
@@ -863,6 +865,7 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
     P.hi = hi;
     uint64_t *d_ids = nullptr;
     uint32_t *d_top = nullptr;
+    int rc = 0;
     if (per_chunk_hi) {
         // A batch: the window logic of enumerateKmers_3minimizers.c skips the
         // k-mers outside the scan's bucket range, so every sample takes the
@@ -871,18 +874,30 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
         const int C = ctx->C;
         MP_TRY(tmp_alloc(ctx, (void **)&d_ids, sizeof(uint64_t) * (C + 1) + sizeof(uint32_t) * (C + 1)));
         d_top = (uint32_t *)(d_ids + (C + 1));
-        MP_HIP(hipMemcpyAsync(d_ids, ctx->chunk_ids.data(), sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice,
-                              ctx->stream));
-        MP_TRY(fill_u32(ctx, d_top, 0, C + 1));
-        P.chunk_ids = d_ids;
-        P.C = C;
-        P.counts = d_top;
-        hipLaunchKernelGGL(kmer_min3_kernel<kModeTop>, dim3((unsigned)((ctx->N + kKBlock - 1) / kKBlock)),
-                           dim3(kKBlock), 0, ctx->stream, P);
-        MP_HIP(hipGetLastError());
-        P.counts = nullptr;
-        P.chunk_hi = d_top;
+        auto ranges = [&]() -> int {
+            MP_HIP(hipMemcpyAsync(d_ids, ctx->chunk_ids.data(), sizeof(uint64_t) * (C + 1), hipMemcpyHostToDevice,
+                                  ctx->stream));
+            MP_TRY(fill_u32(ctx, d_top, 0, C + 1));
+            P.chunk_ids = d_ids;
+            P.C = C;
+            P.counts = d_top;
+            hipLaunchKernelGGL(kmer_min3_kernel<kModeTop>, dim3((unsigned)((ctx->N + kKBlock - 1) / kKBlock)),
+                               dim3(kKBlock), 0, ctx->stream, P);
+            MP_HIP(hipGetLastError());
+            P.counts = nullptr;
+            P.chunk_hi = d_top;
+            return 0;
+        };
+        rc = ranges();
     }
+    if (rc == 0) rc = enumerate_tuples(ctx, s, P, n_out);
+    const int rf = tmp_free(ctx, d_ids);  // on the error paths too
+    return rc ? rc : rf;
+}
+
+// the count, scan and emit launches of enumerate_scan over the prepared parameters
+static int enumerate_tuples(mp_ctx *ctx, int s, KParams &P, uint64_t *n_out)
+{
     P.scan0 = s == 0;
     P.parent = (const uint32_t *)ctx->parent.p;
     P.bad = (const uint8_t *)ctx->bad.p;
@@ -910,7 +925,6 @@ int enumerate_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, uint64_t *n_out
         hipLaunchKernelGGL(kmer_min3_kernel<kModeEmit>, dim3((unsigned)nwg), dim3(kKBlock), 0, ctx->stream, P);
         MP_HIP(hipGetLastError());
     }
-    if (d_ids) MP_TRY(tmp_free(ctx, d_ids));
     ctx->cur = 0;
     *n_out = n;
     return 0;
