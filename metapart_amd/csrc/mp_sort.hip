@@ -1622,6 +1622,9 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
     L.moved = d_moved;
     L.seg = nullptr;
     L.nseg = 0;
+    if (!grid_fits(wins, kLThreads))  // checked before the segment table is allocated
+        return ctx->fail(MP_EINVAL, "local pass of %llu windows exceeds one dispatch: use more scans",
+                         (unsigned long long)wins);
     uint64_t *d_lseg = nullptr;
     if (segs && segs->ns > 0) {
         MP_TRY(tmp_alloc(ctx, (void **)&d_lseg, (segs->ns + 1) * sizeof(uint64_t)));
@@ -1630,9 +1633,6 @@ int radix_sort_tuples(mp_ctx *ctx, uint64_t n, int *passes_out, const GenSource 
         L.seg = d_lseg;
         L.nseg = segs->ns;
     }
-    if (!grid_fits(wins, kLThreads))
-        return ctx->fail(MP_EINVAL, "local pass of %llu windows exceeds one dispatch: use more scans",
-                         (unsigned long long)wins);
     hipEventRecord(ctx->pev[22], ctx->stream);
     hipLaunchKernelGGL(local_rank_kernel, dim3((unsigned)wins), dim3(kLThreads), 0, ctx->stream, L);
     MP_HIP(hipGetLastError());
