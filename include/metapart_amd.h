@@ -260,6 +260,7 @@ typedef struct {
     int engines;              /* concurrent contexts (0: 3) */
     int max_jobs_per_batch;   /* 0: 64; at most MP_MAX_SAMPLES */
     uint64_t hbm_budget;      /* device bytes for all engines (0: 3/4 of free memory) */
+    int min3;                 /* as mp_cfg: 0 every k-mer, 1 three window minimizers per read */
 } mp_server_cfg;
 
 typedef struct {

@@ -70,7 +70,7 @@ class SampleResult(ct.Structure):
 
 class ServerCfg(ct.Structure):
     _fields_ = [("device", ct.c_int), ("k", ct.c_int), ("engines", ct.c_int), ("max_jobs_per_batch", ct.c_int),
-                ("hbm_budget", ct.c_uint64)]
+                ("hbm_budget", ct.c_uint64), ("min3", ct.c_int)]
 
 
 class JobResult(ct.Structure):
@@ -349,9 +349,10 @@ class Context:
 class Server:
     """The served form (mp_server): submit jobs at any time, wait for each."""
 
-    def __init__(self, k: int, engines: int = 3, max_jobs_per_batch: int = 64, hbm_budget: int = 0, device: int = 0):
+    def __init__(self, k: int, engines: int = 3, max_jobs_per_batch: int = 64, hbm_budget: int = 0, device: int = 0,
+                 min3: bool = False):
         cfg = ServerCfg(device=device, k=k, engines=engines, max_jobs_per_batch=max_jobs_per_batch,
-                        hbm_budget=hbm_budget)
+                        hbm_budget=hbm_budget, min3=1 if min3 else 0)
         h = ct.c_void_p()
         rc = lib().mp_server_create(ct.byref(cfg), ct.byref(h))
         if rc:

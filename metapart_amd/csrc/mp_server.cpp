@@ -155,7 +155,7 @@ extern "C" {
 
 int mp_server_create(const mp_server_cfg *cfg, mp_server **out)
 {
-    if (!cfg || !out || cfg->k < 11 || cfg->k > 32) return MP_EINVAL;
+    if (!cfg || !out || cfg->k < 11 || cfg->k > 32 || (cfg->min3 != 0 && cfg->min3 != 1)) return MP_EINVAL;
     auto *s = new mp_server();
     s->cfg = *cfg;
     if (s->cfg.engines <= 0) s->cfg.engines = 3;
@@ -180,6 +180,7 @@ int mp_server_create(const mp_server_cfg *cfg, mp_server **out)
         mp_cfg c{};
         c.device = s->cfg.device;
         c.k = s->cfg.k;
+        c.min3 = s->cfg.min3;
         c.hbm_budget = s->share;
         mp_ctx *ctx = nullptr;
         const int rc = mp_ctx_create(&c, &ctx);

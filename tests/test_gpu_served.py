@@ -1,11 +1,13 @@
 """Served form: jobs submitted over time to the request loop come back with
 the result of a plain run of each sample alone, whether they ran alone or
 batched with others, and the HBM budget turns a job that cannot fit into an
-error for that job only."""
+error for that job only; 3-minimizer jobs are served as plain ones are."""
 import time
 
 import numpy as np
 import pytest
+
+from _helpers import oracle_run
 
 import metapart_amd as mp
 
@@ -22,8 +24,8 @@ def _jobs():
     return [mp.synth(0, seed=900 + i, err_rate=0.01, **sp) for i, sp in enumerate(specs)]
 
 
-def _alone(buf):
-    with mp.Context(k=K) as ctx:
+def _alone(buf, min3=False):
+    with mp.Context(k=K, min3=min3) as ctx:
         ctx.load(buf, [0, buf.nbytes])
         r = mp.partition(ctx, S=1, T=1, want_parent=True, build_output=False)
     return r.W, r.parent.astype(np.uint32)
@@ -69,3 +71,26 @@ def test_served_budget_rejects_only_the_oversized_job():
         assert e.value.code == -2
         res, _ = srv.wait(js)
         assert np.array_equal(res["W"], W0)
+
+
+@pytest.mark.gpu
+def test_served_min3_jobs_equal_alone_and_oracle(tmp_path):
+    """3-minimizer jobs: batched by the server, each equals its run alone, and
+    two of them equal the oracle."""
+    print("gpu served: 3-minimizer jobs", flush=True)
+    specs = [(8000, 6), (2000, 2), (12000, 12), (4000, 3)]
+    bufs = [mp.synth(0, seed=950 + i, err_rate=0.01, n_reads=n, n_genomes=g, genome_len_min=4000,
+                     genome_len_max=20000, read_len=150) for i, (n, g) in enumerate(specs)]
+    alone = [_alone(b, min3=True) for b in bufs]
+    with mp.Server(k=K, engines=1, min3=True) as srv:
+        jobs = [srv.submit(b, want_parent=True) for b in bufs]  # jobs 1-3 arrive while job 0 runs
+        got = [srv.wait(j, want_parent=True) for j in jobs]
+    for (res, par), (W, p0) in zip(got, alone):
+        assert np.array_equal(res["W"], W)
+        assert np.array_equal(par, p0)
+    assert max(res["batch_jobs"] for res, _ in got) >= 2
+    for i in (0, 2):
+        p = str(tmp_path / ("s%d.fastq" % i))
+        open(p, "wb").write(bufs[i].tobytes())
+        ref = oracle_run(p, 1, K, 1, 1, min3=True, out_dir=str(tmp_path / ("o%d" % i)))
+        assert np.array_equal(alone[i][1], ref["parent"].astype(np.uint32))
