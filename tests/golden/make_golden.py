@@ -94,6 +94,11 @@ CASES = [
     ("deep_k16_S2T2", "deep", 4, 16, 2, 2, "se", ""),   # k = 16: even, every sort pass global
     ("many_k28_T4", "many", 8, 28, 1, 4, "se", ""),
     ("many_min3_k31_T2", "many", 8, 31, 1, 2, "se", "min"),
+    ("var_k11_S3", "var", 5, 11, 3, 1, "se", ""),   # the smallest k; an odd chunk count
+    ("tiny_k12", "tiny", 3, 12, 1, 1, "se", ""),
+    ("il_k17_S2", "il150", 7, 17, 2, 1, "il", ""),
+    ("mixed_min3_k21_T3", "mixed", 6, 21, 1, 3, "se", "min"),
+    ("c1_k24_S3T4", "c1", 8, 24, 3, 4, "se", ""),
 ]
 
 
