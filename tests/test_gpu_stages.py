@@ -40,7 +40,14 @@ CASES.append(
     # sort's local pass hands whole groups to the per-group global passes
     ("deep_repeat", dict(n_reads=60000, n_genomes=1, genome_len_min=3000, genome_len_max=3000, read_len=150,
                          err_rate=0.005, seed=12), 4, 31, 1, 1, False, False, 0))
-LONG_GROUPS = {"deep_repeat"}
+CASES.append(
+    # one 400 bp genome at ~60000x: k-mer runs of up to 72 k tuples (past the
+    # filter's 16-bit tallies of a staged run, so evaluated from memory) and
+    # sort groups far past the local pass's window
+    ("ultra_deep", dict(n_reads=160000, n_genomes=1, genome_len_min=400, genome_len_max=400, read_len=150,
+                        err_rate=0.002, seed=13), 4, 31, 1, 1, False, False, 0))
+LONG_GROUPS = {"deep_repeat", "ultra_deep"}
+LONG_RUNS = {"ultra_deep"}
 
 
 @pytest.mark.gpu
@@ -114,6 +121,10 @@ def _check_stages(case, tmp_path):
             if name in LONG_GROUPS:
                 assert st.sort_long_groups > 0
             if s == dump:
+                if name in LONG_RUNS:  # the case holds the runs it is meant to
+                    keys = ref["tuples"][:, :8].copy().view(np.uint64).ravel()
+                    starts = np.concatenate([[0], np.flatnonzero(np.diff(keys)) + 1, [keys.size]])
+                    assert np.diff(starts).max() > 65535
                 tup = ctx.dump_tuples()
                 assert tup.shape == ref["tuples"].shape
                 assert np.array_equal(tup, ref["tuples"])
