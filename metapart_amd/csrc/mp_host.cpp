@@ -151,6 +151,7 @@ extern "C" int mp_index_fastq(int num_chunks, int nPE, int nSE, int nIL, const c
     Sink sink = [&](const mp_fileindex_hdr &h) -> int { return mp_fileindex_write(out, &h, nullptr); };
     int rc = index_core(num_chunks, 0, nSE, fsize, probe, sink);
     if (fclose(out) != 0 && !rc) rc = MP_EIO;
+    if (rc) remove(path.c_str());  // no partial index for a later step to read
     return rc;
 }
 

@@ -114,6 +114,33 @@ def run_chain(kind, case, d, timeout=600, input_path=None):
     return hash_outputs(d)
 
 
+def run_degenerate(kind, case, d, timeout=300):
+    """An input whose metapart step the reference cannot run (golden.json
+    "degenerate"): genFastqIdx and genHistograms of `kind` must succeed; returns
+    the hashes of the in.fastq.* files they wrote and metapart's completed
+    process (not checked here)."""
+    os.makedirs(os.path.join(d, "out", "Rank0"), exist_ok=True)
+    make_input(case["input"], os.path.join(d, "in.fastq"))
+    a = ["0", "0", "1"] if case["mode"] == "il" else ["0", "1", "0"]
+    v, C, K = case["variant"], str(case["C"]), str(case["K"])
+    if kind == "oracle":
+        pre = [[ORACLE_CLI, "genFastqIdx", C] + a + ["in.fastq"], [ORACLE_CLI, "genHistograms" + v, K] + a + ["in.fastq"]]
+        last = [ORACLE_CLI, "metapart" + v, "-o", "./out", "2000000000", K, "1", "1"] + a + ["in.fastq"]
+    else:
+        pre = [[os.path.join(BIN, "genFastqIdx"), C] + a + ["in.fastq"],
+               [os.path.join(BIN, "genHistograms_metapart" + v), K] + a + ["in.fastq"]]
+        last = [os.path.join(BIN, "metapart" + v), "-o", "./out", "2000000000", K, "1", "1"] + a + ["in.fastq"]
+    for c in pre:
+        subprocess.run(c, cwd=d, check=True, stdout=subprocess.DEVNULL, timeout=timeout)
+    h = {}
+    for f in sorted(os.listdir(d)):
+        if f.startswith("in.fastq."):
+            p = os.path.join(d, f)
+            h[f[len("in.fastq."):]] = masked_sha(p) if "idx" in f.lower() else sha(p)
+    r = subprocess.run(last, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    return h, r
+
+
 def diff_hashes(got, want):
     bad = sorted(k for k in set(got) | set(want) if got.get(k) != want.get(k))
     return bad

@@ -55,7 +55,17 @@ SYNTH_INPUTS = {
     # components and singleton k-mers
     "many": ["--preset", "0", "--reads", "20000", "--genomes", "60", "--glen-min", "2000", "--glen-max", "6000",
              "--sigma", "1.0", "--len", "100", "--err", "0.02", "--seed", "33"],
+    # reads of 10-20 bases: no window of k = 21 (the reference's metapart
+    # walks past its range arrays on the empty histogram and crashes)
+    "kmerless": ["--preset", "0", "--reads", "2000", "--genomes", "2", "--len-min", "10", "--len-max", "20",
+                 "--err", "0.01", "--seed", "3"],
 }
+# Inputs whose metapart step the reference cannot run: the hashes of what its
+# genFastqIdx and genHistograms write, and metapart's exit status.
+DEGENERATE = [
+    ("kmerless_k21", "kmerless", 4, 21, "se", ""),
+    ("kmerless_min3_k21", "kmerless", 4, 21, "se", "min"),
+]
 CASES = [
     ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
     ("sample_T1", "reads.fastq", 4, 27, 1, 1, "il", ""),
@@ -242,6 +252,23 @@ def main_viewidx():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def run_degenerate(case, work):
+    name, inp, C, K, mode, variant = case
+    d = os.path.join(work, name)
+    os.makedirs(os.path.join(d, "out", "Rank0"))
+    make_input(inp, os.path.join(d, "in.fastq"))
+    a = ["0", "0", "1"] if mode == "il" else ["0", "1", "0"]
+    q = dict(cwd=d, env=dict(os.environ, OMP_NUM_THREADS="1"), stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    subprocess.run([os.path.join(REF, "genFastqIdx"), str(C)] + a + ["in.fastq"], check=True, **q)
+    subprocess.run([os.path.join(REF, "genHistograms_metapart" + variant), str(K)] + a + ["in.fastq"], check=True, **q)
+    h = {f[len("in.fastq."):]: (masked_sha(os.path.join(d, f)) if "idx" in f.lower() else sha(os.path.join(d, f)))
+         for f in sorted(os.listdir(d)) if f.startswith("in.fastq.")}
+    r = subprocess.run([os.path.join(REF, "metapart" + variant), "-o", "./out", "2000000000", str(K), "1", "1"] + a +
+                       ["in.fastq"], timeout=120, **q)
+    return {"name": name, "input": inp, "C": C, "K": K, "mode": mode, "variant": variant, "sha256": h,
+            "reference_metapart_status": r.returncode}
+
+
 def main():
     if "--large" in sys.argv:
         return main_large()
@@ -258,6 +285,9 @@ def main():
             res["cases"].append({"name": name, "input": inp, "C": C, "K": K, "S": S, "T": T, "mode": mode,
                                  "variant": variant, "sha256": h})
             print(name, len(h), "files")
+        res["degenerate"] = [run_degenerate(c, work) for c in DEGENERATE]
+        for c in res["degenerate"]:
+            print(c["name"], "reference metapart status", c["reference_metapart_status"])
         with open(os.path.join(HERE, "golden.json"), "w") as f:
             json.dump(res, f, indent=1, sort_keys=True)
     finally:

@@ -106,6 +106,26 @@ def test_two_file_paired_end_input_is_refused(tmp_path):
     assert mp.lib().mp_index_fastq(2, 1, 0, 0, arr, 2) == -1
 
 
+def test_index_tool_refuses_empty_input(tmp_path):
+    """An empty FASTQ, on which the reference's genFastqIdx stops at an
+    assertion (genFastqIdx.c:250-251: opening a chunk's file fails): the tool
+    exits with an error and leaves no index behind."""
+    d = str(tmp_path)
+    open(os.path.join(d, "in.fastq"), "wb").close()
+    r = subprocess.run([os.path.join(BIN, "genFastqIdx"), "4", "0", "1", "0", "in.fastq"], cwd=d,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert r.returncode != 0 and "error" in r.stderr
+    assert not os.path.exists(os.path.join(d, "in.fastq.idx"))
+
+
+def test_plan_ranges_refuses_an_empty_histogram():
+    """No k-mers at all: the reference's planning walk writes past its arrays
+    (its metapart crashes on such an input); mp_plan_ranges refuses."""
+    with pytest.raises(mp.MPError) as e:
+        mp.plan_ranges(np.zeros(2 ** 20 + 1, dtype=np.uint64), 1, 1)
+    assert e.value.code == -3
+
+
 def test_index_buffer_equals_index_file(tmp_path):
     buf = mp.synth(1)
     p = os.path.join(str(tmp_path), "in.fastq")

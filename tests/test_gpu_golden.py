@@ -6,9 +6,12 @@ with 'N', variable read lengths, reads at or below k, lower-case and other
 IUPAC letters, the reference's own data/reads.fastq."""
 import pytest
 
-from _helpers import diff_hashes, golden, run_chain
+import os
+
+from _helpers import diff_hashes, golden, run_chain, run_degenerate
 
 CASES = [c for c in golden()["cases"]]
+DEGENERATE = golden()["degenerate"]
 
 
 @pytest.mark.gpu
@@ -47,3 +50,16 @@ def test_gpu_tools_match_reference_with_byte_digits(name, tmp_path, monkeypatch)
     case = next(c for c in CASES if c["name"] == name)
     got = run_chain("gpu", case, str(tmp_path), timeout=300)
     assert diff_hashes(got, case["sha256"]) == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", DEGENERATE, ids=[c["name"] for c in DEGENERATE])
+def test_gpu_tools_on_kmerless_input(case, tmp_path):
+    """No read holds a window of k: the index and the (empty) histograms equal
+    the reference's; metapart, where the reference crashes, exits with an error
+    and writes no output."""
+    print("gpu kmerless case", case["name"], flush=True)
+    got, r = run_degenerate("gpu", case, str(tmp_path))
+    assert diff_hashes(got, case["sha256"]) == []
+    assert r.returncode != 0 and "no k-mers" in r.stdout
+    assert os.listdir(os.path.join(str(tmp_path), "out", "Rank0")) == []

@@ -6,7 +6,7 @@ import os
 
 import pytest
 
-from _helpers import diff_hashes, golden, golden_large, make_large_input, run_chain
+from _helpers import diff_hashes, golden, golden_large, make_large_input, run_chain, run_degenerate
 
 CASES = [c for c in golden()["cases"]]
 
@@ -15,6 +15,22 @@ CASES = [c for c in golden()["cases"]]
 def test_oracle_matches_reference(case, tmp_path):
     got = run_chain("oracle", case, str(tmp_path))
     assert diff_hashes(got, case["sha256"]) == []
+
+
+DEGENERATE = golden()["degenerate"]
+
+
+@pytest.mark.parametrize("case", DEGENERATE, ids=[c["name"] for c in DEGENERATE])
+def test_oracle_on_kmerless_input(case, tmp_path):
+    """No read holds a window of k: the index and the (empty) histograms equal
+    the reference's; metapart, where the reference crashes (it walks past its
+    range arrays on the empty histogram), fails with an error and writes no
+    output."""
+    assert case["reference_metapart_status"] != 0
+    got, r = run_degenerate("oracle", case, str(tmp_path))
+    assert diff_hashes(got, case["sha256"]) == []
+    assert r.returncode != 0 and "no k-mers" in r.stdout
+    assert os.listdir(os.path.join(str(tmp_path), "out", "Rank0")) == []
 
 
 # Workload-size cases (golden_large.json): minutes of CPU time and up to ~60 GB
