@@ -60,6 +60,38 @@ SYNTH_INPUTS = {
     "kmerless": ["--preset", "0", "--reads", "2000", "--genomes", "2", "--len-min", "10", "--len-max", "20",
                  "--err", "0.01", "--seed", "3"],
 }
+
+def random_cases(n=12, seed=777):
+    """Configurations and inputs drawn from a fixed seed (k = 13..32, one to
+    three scans, thread segments dividing the chunk count, plain and
+    3-minimizer, single-end and interleaved, 'N's, mixed letters): inputs
+    rnd<i>, one case each.  The 3-minimizer variant and k = 32 take one scan."""
+    import random
+    rng = random.Random(seed)
+    inputs, cases = {}, []
+    for i in range(n):
+        min3 = rng.random() < 0.3
+        K = rng.randint(13, 31 if min3 else 32)
+        C = rng.choice([4, 6, 8, 12, 16])
+        T = rng.choice([t for t in (1, 2, 3, 4, 6, 8) if C % t == 0])
+        S = 1 if (min3 or K == 32) else rng.randint(1, 3)
+        il = rng.random() < 0.2
+        lo = rng.randint(K + 5, 130)
+        hi = rng.randint(lo, 200)
+        a = ["--preset", "0", "--reads", str(2 * rng.randint(1000, 6000)), "--genomes", str(rng.randint(1, 20)),
+             "--glen-min", "3000", "--glen-max", str(rng.randint(3000, 30000)), "--len-min", str(lo), "--len-max",
+             str(hi), "--err", str(rng.choice([0.005, 0.01, 0.02])), "--nrate", str(rng.choice([0, 0.002, 0.005])),
+             "--seed", str(500 + i)]
+        if rng.random() < 0.25:
+            a += ["--mix", "0.05"]
+        if il:
+            a += ["--interleaved"]
+        inputs["rnd%d" % i] = a
+        cases.append(("rnd%d_k%d_S%dT%d%s%s" % (i, K, S, T, "_min3" if min3 else "", "_il" if il else ""), "rnd%d" % i,
+                      C, K, S, T, "il" if il else "se", "min" if min3 else ""))
+    return inputs, cases
+
+
 # Inputs whose metapart step the reference cannot run: the hashes of what its
 # genFastqIdx and genHistograms write, and metapart's exit status.
 DEGENERATE = [
@@ -110,6 +142,9 @@ CASES = [
     ("mixed_min3_k21_T3", "mixed", 6, 21, 1, 3, "se", "min"),
     ("c1_k24_S3T4", "c1", 8, 24, 3, 4, "se", ""),
 ]
+_RIN, RANDOM_CASES = random_cases()
+SYNTH_INPUTS.update(_RIN)
+CASES += RANDOM_CASES
 
 
 # Workload-size cases (minutes of reference CPU time each; golden_large.json).
@@ -280,7 +315,13 @@ def main():
     try:
         res = {"inputs": SYNTH_INPUTS, "cases": []}
         for case in CASES:
-            h = run_reference(case, work)
+            try:
+                h = run_reference(case, work)
+            except subprocess.CalledProcessError as e:
+                if case not in RANDOM_CASES:
+                    raise
+                print(case[0], "dropped: the reference fails on it:", e.returncode)  # a plan it cannot run
+                continue
             name, inp, C, K, S, T, mode, variant = case
             res["cases"].append({"name": name, "input": inp, "C": C, "K": K, "S": S, "T": T, "mode": mode,
                                  "variant": variant, "sha256": h})
