@@ -61,7 +61,7 @@ SYNTH_INPUTS = {
                  "--err", "0.01", "--seed", "3"],
 }
 
-def random_cases(n=12, seed=777):
+def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500):
     """Configurations and inputs drawn from a fixed seed (k = 13..32, one to
     three scans, thread segments dividing the chunk count, plain and
     3-minimizer, single-end and interleaved, 'N's, mixed letters): inputs
@@ -78,17 +78,17 @@ def random_cases(n=12, seed=777):
         il = rng.random() < 0.2
         lo = rng.randint(K + 5, 130)
         hi = rng.randint(lo, 200)
-        a = ["--preset", "0", "--reads", str(2 * rng.randint(1000, 6000)), "--genomes", str(rng.randint(1, 20)),
+        a = ["--preset", "0", "--reads", str(2 * rng.randint(*reads)), "--genomes", str(rng.randint(1, 20)),
              "--glen-min", "3000", "--glen-max", str(rng.randint(3000, 30000)), "--len-min", str(lo), "--len-max",
              str(hi), "--err", str(rng.choice([0.005, 0.01, 0.02])), "--nrate", str(rng.choice([0, 0.002, 0.005])),
-             "--seed", str(500 + i)]
+             "--seed", str(seed0 + i)]
         if rng.random() < 0.25:
             a += ["--mix", "0.05"]
         if il:
             a += ["--interleaved"]
-        inputs["rnd%d" % i] = a
-        cases.append(("rnd%d_k%d_S%dT%d%s%s" % (i, K, S, T, "_min3" if min3 else "", "_il" if il else ""), "rnd%d" % i,
-                      C, K, S, T, "il" if il else "se", "min" if min3 else ""))
+        inputs["%s%d" % (prefix, i)] = a
+        cases.append(("%s%d_k%d_S%dT%d%s%s" % (prefix, i, K, S, T, "_min3" if min3 else "", "_il" if il else ""),
+                      "%s%d" % (prefix, i), C, K, S, T, "il" if il else "se", "min" if min3 else ""))
     return inputs, cases
 
 
@@ -143,6 +143,10 @@ CASES = [
     ("c1_k24_S3T4", "c1", 8, 24, 3, 4, "se", ""),
 ]
 _RIN, RANDOM_CASES = random_cases()
+# and 6 of 20-80 k reads: more sort and filter tiles a segment
+_RIN2, _RC2 = random_cases(n=6, seed=9191, prefix="rndL", reads=(10000, 40000), seed0=600)
+_RIN.update(_RIN2)
+RANDOM_CASES += _RC2
 SYNTH_INPUTS.update(_RIN)
 CASES += RANDOM_CASES
 
