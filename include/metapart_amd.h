@@ -145,7 +145,9 @@ int mp_count_chunks_noalloc(int device, int k, int min3, const void *d_fq, uint6
 /* Uploads the FASTQ bytes of a run (C chunks as above) and parses every
  * record; chunk_first_id[c] is the read id of chunk c's first record
  * (KmerIdx readId/readIdExtra), chunk_first_id[C] the read count.
- * Reads stay resident across scans.
+ * Reads stay resident across scans.  An odd read count loads and can be
+ * counted (as genHistograms counts it); mp_scan refuses it with MP_EINVAL
+ * (the reference's pair step reads id N of such a run and crashes).
  * Replaces loadChunk / getNextRead (metapart.c:296-500) and the scan-0
  * parent initialisation (metapart.c:3933-3954). */
 int mp_load_reads(mp_ctx *ctx, const char *fq, uint64_t nbytes, const uint64_t *chunk_ofs,

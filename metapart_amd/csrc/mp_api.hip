@@ -350,6 +350,10 @@ int mp_scan(mp_ctx *ctx, int s, uint64_t lo, uint64_t hi, const uint64_t *seg_bo
 {
     if (!ctx || !seg_bounds || T <= 0) return MP_EINVAL;
     if (ctx->N == 0) return ctx->fail(MP_ESTATE, "mp_scan before mp_load_reads");
+    // the reference's genHistograms counts an odd number of reads, but its
+    // pair step reads id N of an odd count and its metapart crashes there
+    if (ctx->N & 1)
+        return ctx->fail(MP_EINVAL, "odd read count %llu (the pair step reads id N)", (unsigned long long)ctx->N);
     if (s != ctx->scans_done) return ctx->fail(MP_ESTATE, "scan %d out of order (next is %d)", s, ctx->scans_done);
     if (seg_bounds[0] != lo || seg_bounds[T] != hi) return ctx->fail(MP_EINVAL, "segments must span [lo, hi)");
     for (int t = 0; t < T; t++)

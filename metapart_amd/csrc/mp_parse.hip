@@ -218,8 +218,7 @@ int parse_reads(mp_ctx *ctx, const uint64_t *chunk_ofs, const uint64_t *chunk_fi
                          (unsigned long long)n_nl);
     const uint64_t N = n_nl / 4;
     if (N >= 0xFFFFFFFFull) return ctx->fail(MP_EINVAL, "too many reads for 32-bit ids");
-    if (N & 1) return ctx->fail(MP_EINVAL, "odd read count %llu (the pair step reads id N)", (unsigned long long)N);
-    ctx->N = N;
+    ctx->N = N;  // an odd count loads and counts; mp_scan refuses it
 
     MP_TRY(ensure(ctx, ctx->rec_off, (N + 1) * sizeof(uint64_t)));
     MP_TRY(ensure(ctx, ctx->seq_off, N * sizeof(uint64_t)));

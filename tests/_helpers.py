@@ -41,8 +41,16 @@ def make_input(name, dst):
         shutil.copy(os.path.join(GOLDEN_DIR, name), dst)
         os.chmod(dst, 0o644)
     else:
-        args = golden()["inputs"][name]
+        args = list(golden()["inputs"][name])
+        keep = None
+        if "--keep-records" in args:  # the generator writes even counts: cut its file
+            i = args.index("--keep-records")
+            keep = int(args[i + 1])
+            del args[i:i + 2]
         subprocess.run([os.path.join(BIN, "mp_synth")] + args + ["-o", dst], check=True)
+        if keep is not None:
+            lines = open(dst, "rb").read().split(b"\n")
+            open(dst, "wb").write(b"\n".join(lines[:4 * keep]) + b"\n")
 
 
 def masked_sha(path):

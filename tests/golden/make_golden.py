@@ -59,6 +59,9 @@ SYNTH_INPUTS = {
     # walks past its range arrays on the empty histogram and crashes)
     "kmerless": ["--preset", "0", "--reads", "2000", "--genomes", "2", "--len-min", "10", "--len-max", "20",
                  "--err", "0.01", "--seed", "3"],
+    # 2001 reads: an odd count (the reference counts it; its metapart crashes)
+    "odd2001": ["--preset", "0", "--reads", "2002", "--genomes", "2", "--len", "100", "--err", "0.01", "--seed", "5",
+                "--keep-records", "2001"],
 }
 
 def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500):
@@ -95,8 +98,9 @@ def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500):
 # Inputs whose metapart step the reference cannot run: the hashes of what its
 # genFastqIdx and genHistograms write, and metapart's exit status.
 DEGENERATE = [
-    ("kmerless_k21", "kmerless", 4, 21, "se", ""),
-    ("kmerless_min3_k21", "kmerless", 4, 21, "se", "min"),
+    ("kmerless_k21", "kmerless", 4, 21, "se", "", "no k-mers"),
+    ("kmerless_min3_k21", "kmerless", 4, 21, "se", "min", "no k-mers"),
+    ("odd2001_k27", "odd2001", 4, 27, "se", "", "odd read count"),
 ]
 CASES = [
     ("sample_T2", "reads.fastq", 4, 27, 1, 2, "il", ""),   # the reference's own smoke run (test_script.sh:20)
@@ -189,8 +193,16 @@ def make_input(name, dst):
     if name.endswith(".fastq"):
         shutil.copy(os.path.join(HERE, name), dst)
     else:
-        args = SYNTH_INPUTS[name] if name in SYNTH_INPUTS else LARGE_INPUTS[name]
+        args = list(SYNTH_INPUTS[name] if name in SYNTH_INPUTS else LARGE_INPUTS[name])
+        keep = None
+        if "--keep-records" in args:  # the generator writes even counts: cut its file
+            i = args.index("--keep-records")
+            keep = int(args[i + 1])
+            del args[i:i + 2]
         subprocess.run([SYNTH] + args + ["-o", dst], check=True)
+        if keep is not None:
+            lines = open(dst, "rb").read().split(b"\n")
+            open(dst, "wb").write(b"\n".join(lines[:4 * keep]) + b"\n")
 
 
 def hash_outputs(d, K, variant):
@@ -292,7 +304,7 @@ def main_viewidx():
 
 
 def run_degenerate(case, work):
-    name, inp, C, K, mode, variant = case
+    name, inp, C, K, mode, variant, refusal = case
     d = os.path.join(work, name)
     os.makedirs(os.path.join(d, "out", "Rank0"))
     make_input(inp, os.path.join(d, "in.fastq"))
@@ -305,7 +317,7 @@ def run_degenerate(case, work):
     r = subprocess.run([os.path.join(REF, "metapart" + variant), "-o", "./out", "2000000000", str(K), "1", "1"] + a +
                        ["in.fastq"], timeout=120, **q)
     return {"name": name, "input": inp, "C": C, "K": K, "mode": mode, "variant": variant, "sha256": h,
-            "reference_metapart_status": r.returncode}
+            "reference_metapart_status": r.returncode, "refusal": refusal}
 
 
 def main():

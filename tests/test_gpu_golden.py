@@ -54,12 +54,12 @@ def test_gpu_tools_match_reference_with_byte_digits(name, tmp_path, monkeypatch)
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", DEGENERATE, ids=[c["name"] for c in DEGENERATE])
-def test_gpu_tools_on_kmerless_input(case, tmp_path):
-    """No read holds a window of k: the index and the (empty) histograms equal
-    the reference's; metapart, where the reference crashes, exits with an error
-    and writes no output."""
-    print("gpu kmerless case", case["name"], flush=True)
+def test_gpu_tools_on_input_the_reference_cannot_partition(case, tmp_path):
+    """No read holds a window of k, or the read count is odd: the index and
+    the histograms equal the reference's; metapart, where the reference
+    crashes, exits with an error and writes no output."""
+    print("gpu degenerate case", case["name"], flush=True)
     got, r = run_degenerate("gpu", case, str(tmp_path))
     assert diff_hashes(got, case["sha256"]) == []
-    assert r.returncode != 0 and "no k-mers" in r.stdout
+    assert r.returncode != 0 and case["refusal"] in r.stdout
     assert os.listdir(os.path.join(str(tmp_path), "out", "Rank0")) == []

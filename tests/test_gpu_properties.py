@@ -56,11 +56,14 @@ def test_error_paths_return_codes():
     print("gpu error paths", flush=True)
     good = mp.synth(0, n_reads=2000).tobytes()
     with mp.Context(k=27) as ctx:
-        # odd record count
+        # odd record count: loads and counts (as the reference's genHistograms
+        # does), refused by the scan (the reference's pair step crashes on it)
         rec = good.split(b"\n@")
         odd = b"\n@".join(rec[:-1]) + b"\n"
+        ctx.load(odd, [0, len(odd)])
+        hist, _, _, _ = ctx.count()
         with pytest.raises(mp.MPError) as e:
-            ctx.load(odd, [0, len(odd)])
+            ctx.scan(0, 0, mp.NBUCKETS, [0, mp.NBUCKETS], 0)
         assert e.value.code == -1
         # chunk boundary inside a record
         with pytest.raises(mp.MPError) as e:

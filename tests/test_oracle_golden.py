@@ -21,15 +21,16 @@ DEGENERATE = golden()["degenerate"]
 
 
 @pytest.mark.parametrize("case", DEGENERATE, ids=[c["name"] for c in DEGENERATE])
-def test_oracle_on_kmerless_input(case, tmp_path):
-    """No read holds a window of k: the index and the (empty) histograms equal
-    the reference's; metapart, where the reference crashes (it walks past its
-    range arrays on the empty histogram), fails with an error and writes no
+def test_oracle_on_input_the_reference_cannot_partition(case, tmp_path):
+    """No read holds a window of k, or the read count is odd: the index and
+    the histograms equal the reference's; metapart, where the reference
+    crashes (it walks past its range arrays on an empty histogram; its pair
+    step reads id N of an odd count), fails with an error and writes no
     output."""
     assert case["reference_metapart_status"] != 0
     got, r = run_degenerate("oracle", case, str(tmp_path))
     assert diff_hashes(got, case["sha256"]) == []
-    assert r.returncode != 0 and "no k-mers" in r.stdout
+    assert r.returncode != 0 and case["refusal"] in r.stdout
     assert os.listdir(os.path.join(str(tmp_path), "out", "Rank0")) == []
 
 
