@@ -52,6 +52,27 @@ def test_gpu_tools_match_reference_with_byte_digits(name, tmp_path, monkeypatch)
     assert diff_hashes(got, case["sha256"]) == []
 
 
+# Sort shapes a caller or the autotuner can choose (mp_cfg.sort_tile: 512 x 6,
+# 512 x 8, 512 x 12 and 1024 x 8 tuples a tile), tiles taken in launch order
+# instead of one contiguous range per XCD, and every pass global (no in-place
+# local pass), each on golden cases it does not otherwise meet.
+SORT_VARIANTS = [("MP_SORT_TILE", "3072", "rnd6_k27_S3T4"), ("MP_SORT_TILE", "4096", "rndL3_k21_S1T1"),
+                 ("MP_SORT_TILE", "6144", "c1_S3T2_k31"), ("MP_SORT_TILE", "8192", "gcrep_k31_S4T2"),
+                 ("MP_SORT_TILE", "4096", "rndL5_k24_S1T2_min3"), ("MP_SORT_XCD", "0", "small_k31_S2T2"),
+                 ("MP_SORT_GLOBAL", "1", "mixed_k31"), ("MP_SORT_GLOBAL", "1", "rnd7_k29_S3T2")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("var", SORT_VARIANTS, ids=["%s=%s-%s" % v for v in SORT_VARIANTS])
+def test_gpu_tools_match_reference_with_sort_variants(var, tmp_path, monkeypatch):
+    env, val, name = var
+    print("gpu golden case (%s=%s)" % (env, val), name, flush=True)
+    monkeypatch.setenv(env, val)
+    case = next(c for c in CASES if c["name"] == name)
+    got = run_chain("gpu", case, str(tmp_path), timeout=300)
+    assert diff_hashes(got, case["sha256"]) == []
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", DEGENERATE, ids=[c["name"] for c in DEGENERATE])
 def test_gpu_tools_on_input_the_reference_cannot_partition(case, tmp_path):
