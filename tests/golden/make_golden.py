@@ -64,7 +64,8 @@ SYNTH_INPUTS = {
                 "--keep-records", "2001"],
 }
 
-def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500):
+def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500, cs=(4, 6, 8, 12, 16),
+                 ts=(1, 2, 3, 4, 6, 8)):
     """Configurations and inputs drawn from a fixed seed (k = 13..32, one to
     three scans, thread segments dividing the chunk count, plain and
     3-minimizer, single-end and interleaved, 'N's, mixed letters): inputs
@@ -75,8 +76,8 @@ def random_cases(n=12, seed=777, prefix="rnd", reads=(1000, 6000), seed0=500):
     for i in range(n):
         min3 = rng.random() < 0.3
         K = rng.randint(13, 31 if min3 else 32)
-        C = rng.choice([4, 6, 8, 12, 16])
-        T = rng.choice([t for t in (1, 2, 3, 4, 6, 8) if C % t == 0])
+        C = rng.choice(list(cs))
+        T = rng.choice([t for t in ts if C % t == 0])
         S = 1 if (min3 or K == 32) else rng.randint(1, 3)
         il = rng.random() < 0.2
         lo = rng.randint(K + 5, 130)
@@ -151,6 +152,10 @@ _RIN, RANDOM_CASES = random_cases()
 _RIN2, _RC2 = random_cases(n=6, seed=9191, prefix="rndL", reads=(10000, 40000), seed0=600)
 _RIN.update(_RIN2)
 RANDOM_CASES += _RC2
+# and 6 with 16 or 32 chunks over 8 or 16 thread segments
+_RIN3, _RC3 = random_cases(n=6, seed=4711, prefix="rndT", reads=(5000, 15000), seed0=700, cs=(16, 32), ts=(8, 16))
+_RIN.update(_RIN3)
+RANDOM_CASES += _RC3
 SYNTH_INPUTS.update(_RIN)
 CASES += RANDOM_CASES
 
