@@ -226,9 +226,20 @@ done:
 static const int CMAP[4] = {0, 1, 3, 2};  /* countMmers.c:85 */
 static const int RCMAP[4] = {3, 2, 0, 1}; /* countMmers.c:86 */
 static inline uint64_t kmask(int K) { return 2 * K >= 64 ? 0ull : (1ull << (2 * K)) - 1; } /* x86 shift semantics at K=32 */
-static inline uint64_t mmer_prefix(uint64_t kL, int K) { return kL >> ((K - 10) << 1); } /* countMmers.c:60-81, m = 10 */
+/* ORC_MUTANT (0 here) builds a copy with one planted mistake, which the golden
+ * pins must catch (tests/test_oracle_mutants.py): 1 the pair step's size
+ * gate asking both components to be small, 2 the top bit of the next character's class dropped from
+ * the adj byte, 3 the filter's ratio at 0.25, 4 a 9-base m-mer prefix */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+static inline uint64_t mmer_prefix(uint64_t kL, int K) { return kL >> ((K - (ORC_MUTANT == 4 ? 9 : 10)) << 1); } /* countMmers.c:60-81, m = 10 */
 static inline char rcchar(char c) { return "TGAC   N"[(c >> 1) & 7]; } /* enumerateKmers.c:237 */
-static inline uint8_t adj_fwd(char prev, char next) { return (uint8_t)(((prev << 2) & 0xF8) | ((next >> 1) & 0x7)); }
+static inline uint8_t adj_fwd(char prev, char next)
+{
+    if (ORC_MUTANT == 2) return (uint8_t)(((prev << 2) & 0xF8) | ((next >> 1) & 0x3));
+    return (uint8_t)(((prev << 2) & 0xF8) | ((next >> 1) & 0x7));
+}
 
 /* plain counter: countMmers.c:83-217 */
 static int count_plain(const char *s, int L, int K, uint32_t *counts)
@@ -938,7 +949,9 @@ static void mp_filter(mp_t *m, int scan, uint8_t *st, const uint64_t *lrc, const
                         if (nIn > 1 && nOut > 1) { memset(t, 0, 8); continue; }
                         int bad = 0;
                         if ((nIn > 1 && bIdx != maxIn) || (nOut > 1 && aIdx != maxOut)) bad = 1;
-                        else if ((double)sInR > 0.2 || (double)sOutR > 0.2) bad = 1;
+                        else if ((double)sInR > (ORC_MUTANT == 3 ? 0.25 : 0.2) ||
+                                 (double)sOutR > (ORC_MUTANT == 3 ? 0.25 : 0.2))
+                            bad = 1;
                         if (bad) {
                             uint64_t id = t_label(t);
                             m->parent[id] = ORC_BAD;
@@ -1021,7 +1034,8 @@ static void mp_pair_union(mp_t *m, uint32_t upto)
         if (m->parent[a] == ORC_BAD) m->parent[a] = a;
         if (m->parent[b] == ORC_BAD) m->parent[b] = b;
         uint64_t ra = find_root(m->parent, a), rb = find_root(m->parent, b);
-        if (m->sizes[ra] <= upto || m->sizes[rb] <= upto) {
+        const int small_a = m->sizes[ra] <= upto, small_b = m->sizes[rb] <= upto;
+        if (ORC_MUTANT == 1 ? (small_a && small_b) : (small_a || small_b)) {
             if (ra != rb) {
                 if (ra > rb) { m->parent[rb] = ra; m->sizes[ra] += m->sizes[rb]; m->sizes[rb] = 0; }
                 else { m->parent[ra] = rb; m->sizes[rb] += m->sizes[ra]; m->sizes[ra] = 0; }

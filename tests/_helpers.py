@@ -95,10 +95,11 @@ def make_large_input(name, dst):
     subprocess.run([os.path.join(BIN, "mp_synth")] + args + ["-o", dst], check=True)
 
 
-def run_chain(kind, case, d, timeout=600, input_path=None):
+def run_chain(kind, case, d, timeout=600, input_path=None, cli=ORACLE_CLI):
     """Runs genFastqIdx, genHistograms_metapart[min], metapart[min] of `kind`
-    ('oracle' = CPU restatement, 'gpu' = this build's tools) in directory d,
-    on the case's input (or on input_path, hard-linked into d)."""
+    ('oracle' = CPU restatement, or the oracle build at `cli`; 'gpu' = this
+    build's tools) in directory d, on the case's input (or on input_path,
+    hard-linked into d)."""
     os.makedirs(os.path.join(d, "out", "Rank0"), exist_ok=True)
     if input_path:
         os.link(input_path, os.path.join(d, "in.fastq"))
@@ -108,9 +109,9 @@ def run_chain(kind, case, d, timeout=600, input_path=None):
     v = case["variant"]
     C, K, S, T = str(case["C"]), str(case["K"]), str(case["S"]), str(case["T"])
     if kind == "oracle":
-        cmds = [[ORACLE_CLI, "genFastqIdx", C] + a + ["in.fastq"],
-                [ORACLE_CLI, "genHistograms" + v, K] + a + ["in.fastq"],
-                [ORACLE_CLI, "metapart" + v, "-o", "./out", "2000000000", K, S, T] + a + ["in.fastq"]]
+        cmds = [[cli, "genFastqIdx", C] + a + ["in.fastq"],
+                [cli, "genHistograms" + v, K] + a + ["in.fastq"],
+                [cli, "metapart" + v, "-o", "./out", "2000000000", K, S, T] + a + ["in.fastq"]]
     else:
         cmds = [[os.path.join(BIN, "genFastqIdx"), C] + a + ["in.fastq"],
                 [os.path.join(BIN, "genHistograms_metapart" + v), K] + a + ["in.fastq"],
